@@ -1,0 +1,614 @@
+// eri_body.hpp -- bodies of the shell-quartet ERI kernels.
+//
+// Two kernel families, both McMurchie-Davidson with per-primitive-quartet
+// Boys + R recursion and a two-stage Hermite -> Cartesian contraction:
+//
+//  * family A ("one lane, one shell quartet"): every lane owns one shell
+//    quartet of a low-L class; all primitive loops, E tables, R_tuv and the
+//    output block live in registers.  Used for classes whose block and R fit
+//    the FP64 register budget (see use_family_a()).
+//
+//  * family B ("one wavefront, one shell quartet"): the 64 lanes of a
+//    wavefront cooperate on one shell quartet.  Per primitive quartet the
+//    Boys values and the R_tuv levels are built lane-parallel in LDS, then
+//    the ket contraction g[tuv][cd] and the bra contraction acc[ab][cd] are
+//    spread over lanes.  The ket components are processed in chunks so the
+//    LDS footprint stays under 64 KiB even for (ff|ff).
+//
+// Bodies are __host__ __device__ so the identical code also builds into a
+// CPU emulation library (lane loop with nlanes = 1) used by the CPU tests.
+#pragma once
+#include <stdint.h>
+#include <type_traits>
+#include <utility>
+
+#include "eri_types.hpp"
+#include "md_core.hpp"
+
+namespace hfx {
+
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+#else
+inline void atomic_add(double* p, double v) { *p += v; }
+#endif
+
+constexpr double TWO_PI_POW_2_5 = 34.98683665524972497;   // 2 pi^{5/2}
+
+template <int Begin, int End, class F>
+HFX_HD __forceinline__ void static_for(F&& f) {
+    if constexpr (Begin < End) {
+        f(std::integral_constant<int, Begin>{});
+        static_for<Begin + 1, End>(f);
+    }
+}
+
+// Family A is used while the output block, R_tuv and the E tables stay within
+// about 96 FP64 registers' worth of values.
+__host__ __device__ constexpr bool use_family_a(int la, int lb, int lc, int ld) {
+    int L = la + lb + lc + ld;
+    int nout = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+    return L <= 4 && nout + nherm(L) <= 72;
+}
+
+// ------------------------------------------------------------------ epilogue
+// A shell quartet block v[a][b][c][d] (already scaled to final values) for the
+// kernel-order shells (s1 s2 | s3 s4) is consumed in one of three ways.  The
+// accessor `val(a,b,c,d)` returns the block value in FP64.
+//
+// Degeneracy for EPI_JK: with f = 1/|stabiliser| of the shell quartet under
+// the 8-fold permutation group, summing all 8 images over the full block is
+//   Jacc_ab += 2f sum_cd v Dj_cd ; Jacc_cd += 2f sum_ab v Dj_ab
+//   Kacc_ac += f sum_bd v D_bd ; Kacc_bc += f sum_ad v D_ad
+//   Kacc_ad += f sum_bc v D_bc ; Kacc_bd += f sum_ac v D_ac
+// and J = Jacc + Jacc^T, K = Kacc + Kacc^T afterwards.
+struct QuartetCtx {
+    int fn1, fn2, fn3, fn4;
+    bool d12, d34, same;   // s1 == s2, s3 == s4, bra pair == ket pair
+};
+
+// Canonical filter: each unique function quartet is emitted exactly once.
+HFX_HD __forceinline__ bool keep_unique(const QuartetCtx& q, int mu, int nu, int la, int si) {
+    if (q.d12 && mu < nu) return false;
+    if (q.d34 && la < si) return false;
+    if (q.same && pair_of(mu, nu) < pair_of(la, si)) return false;
+    return true;
+}
+
+HFX_HD __forceinline__ bool mask_pass(const EpiArgs& ep, int mu, int nu, int la, int si) {
+    return ep.bounds[pair_of(mu, nu)] * ep.bounds[pair_of(la, si)] >= ep.threshold;
+}
+
+// ------------------------------------------------------------------ family A
+template <int LA, int LB, int LC, int LD, class Real>
+struct FamA {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+    static constexpr int NAB = NA * NB, NCD = NC * ND, NOUT = NAB * NCD;
+
+    // acc[ab*NCD + cd] += sum_tuv Ebra_ab(tuv) sum_{tau nu phi} Eket_cd(tau nu phi) R(t+tau, u+nu, v+phi)
+    static HFX_HD __forceinline__ void contract(Real* acc, const Herm1D<LA, LB, Real>& ex,
+                                                const Herm1D<LA, LB, Real>& ey, const Herm1D<LA, LB, Real>& ez,
+                                                const Herm1D<LC, LD, Real>& fx, const Herm1D<LC, LD, Real>& fy,
+                                                const Herm1D<LC, LD, Real>& fz, const Real* R) {
+        static_for<0, NC>([&](auto ci) {
+            static_for<0, ND>([&](auto di) {
+                constexpr int c = decltype(ci)::value, d = decltype(di)::value;
+                constexpr int cx = cart_x(LC, c), cy = cart_y(LC, c), cz = cart_z(LC, c);
+                constexpr int dx = cart_x(LD, d), dy = cart_y(LD, d), dz = cart_z(LD, d);
+                Real g[nherm(LAB)];
+                static_for<0, LAB + 1>([&](auto ti) {
+                    static_for<0, LAB + 1 - decltype(ti)::value>([&](auto ui) {
+                        static_for<0, LAB + 1 - decltype(ti)::value - decltype(ui)::value>([&](auto vi) {
+                            constexpr int t = decltype(ti)::value, u = decltype(ui)::value, v = decltype(vi)::value;
+                            Real s = Real(0);
+                            static_for<0, cx + dx + 1>([&](auto tai) {
+                                constexpr int ta = decltype(tai)::value;
+                                Real sy = Real(0);
+                                static_for<0, cy + dy + 1>([&](auto nui) {
+                                    constexpr int nn = decltype(nui)::value;
+                                    Real sz = Real(0);
+                                    static_for<0, cz + dz + 1>([&](auto phi) {
+                                        constexpr int ph = decltype(phi)::value;
+                                        sz += fz.e[cz][dz][ph] * R[hidx(t + ta, u + nn, v + ph)];
+                                    });
+                                    sy += fy.e[cy][dy][nn] * sz;
+                                });
+                                s += fx.e[cx][dx][ta] * sy;
+                            });
+                            g[hidx(t, u, v)] = s;
+                        });
+                    });
+                });
+                static_for<0, NA>([&](auto ai) {
+                    static_for<0, NB>([&](auto bi) {
+                        constexpr int a = decltype(ai)::value, b = decltype(bi)::value;
+                        constexpr int ax = cart_x(LA, a), ay = cart_y(LA, a), az = cart_z(LA, a);
+                        constexpr int bx = cart_x(LB, b), by = cart_y(LB, b), bz = cart_z(LB, b);
+                        Real s = Real(0);
+                        static_for<0, ax + bx + 1>([&](auto ti) {
+                            constexpr int t = decltype(ti)::value;
+                            Real sy = Real(0);
+                            static_for<0, ay + by + 1>([&](auto ui) {
+                                constexpr int u = decltype(ui)::value;
+                                Real sz = Real(0);
+                                static_for<0, az + bz + 1>([&](auto vi) {
+                                    constexpr int v = decltype(vi)::value;
+                                    sz += ez.e[az][bz][v] * g[hidx(t, u, v)];
+                                });
+                                sy += ey.e[ay][by][u] * sz;
+                            });
+                            s += ex.e[ax][bx][t] * sy;
+                        });
+                        acc[(a * NB + b) * NCD + c * ND + d] += s;
+                    });
+                });
+            });
+        });
+    }
+
+    // Full primitive loop for one shell quartet; acc must be zeroed.
+    static HFX_HD __forceinline__ void quartet(const PairHdr& hb, const PairHdr& hk, const PrimPair<Real>* pp,
+                                               const Real* tab, Real* acc) {
+        for (int i = 0; i < hb.nprim; ++i) {
+            const PrimPair<Real> b = pp[hb.prim_off + i];
+            Herm1D<LA, LB, Real> ex, ey, ez;
+            ex.build(b.i2p, b.PA[0], b.PB[0]);
+            ey.build(b.i2p, b.PA[1], b.PB[1]);
+            ez.build(b.i2p, b.PA[2], b.PB[2]);
+            for (int j = 0; j < hk.nprim; ++j) {
+                const PrimPair<Real> k = pp[hk.prim_off + j];
+                Herm1D<LC, LD, Real> fx, fy, fz;
+                fx.build(k.i2p, k.PA[0], k.PB[0]);
+                fy.build(k.i2p, k.PA[1], k.PB[1]);
+                fz.build(k.i2p, k.PA[2], k.PB[2]);
+                // (-1)^{tau+nu+phi}: negate odd Hermite orders of the ket tables
+                static_for<0, LC + 1>([&](auto ci) {
+                    static_for<0, LD + 1>([&](auto di) {
+                        static_for<0, LCD + 1>([&](auto ti) {
+                            constexpr int c = decltype(ci)::value, d = decltype(di)::value, t = decltype(ti)::value;
+                            if constexpr (t % 2 == 1) {
+                                fx.e[c][d][t] = -fx.e[c][d][t];
+                                fy.e[c][d][t] = -fy.e[c][d][t];
+                                fz.e[c][d][t] = -fz.e[c][d][t];
+                            }
+                        });
+                    });
+                });
+                Real s = b.p + k.p;
+                Real inv = Real(1) / s;
+                Real rho = b.p * k.p * inv;
+                Real X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
+                Real T = rho * (X * X + Y * Y + Z * Z);
+                Real F[L + 1];
+                boys_fm<L, Real>(T, tab, F);
+                Real pref = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
+                Real a[L + 1];
+                Real m2r = Real(-2) * rho;
+                Real f = pref;
+#pragma unroll
+                for (int n = 0; n <= L; ++n) {
+                    a[n] = f * F[n];
+                    f *= m2r;
+                }
+                Real R[nherm(L)];
+                hermite_R<L, Real>(a, X, Y, Z, R);
+                contract(acc, ex, ey, ez, fx, fy, fz, R);
+            }
+        }
+    }
+};
+
+// Family-A epilogue over a register block acc[NOUT] (raw, unscaled).
+template <int NA, int NB, int NC, int ND, class Real>
+HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep, const Real* acc) {
+    constexpr int NCD = NC * ND;
+    const double* fs = ep.fscale;
+    if (ep.mode == EPI_UNIQUE) {
+#pragma unroll
+        for (int e = 0; e < NA * NB * NCD; ++e) {
+            const int a = e / (NB * NCD), b = (e / NCD) % NB, c = (e / ND) % NC, d = e % ND;
+            const int mu = q.fn1 + a, nu = q.fn2 + b, la = q.fn3 + c, si = q.fn4 + d;
+            if (!keep_unique(q, mu, nu, la, si)) continue;
+            const int64_t IJ = pair_of(mu, nu), KL = pair_of(la, si);
+            if (ep.bounds[IJ] * ep.bounds[KL] < ep.threshold) continue;
+            ep.out[quartet_of(IJ, KL)] = (double)acc[e] * (fs[mu] * fs[nu] * fs[la] * fs[si]);
+        }
+        return;
+    }
+    if (ep.mode == EPI_DIAG) {
+        // bra pair == ket pair; (mu nu|mu nu) sits at a == c, b == d
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int mu = q.fn1 + a, nu = q.fn2 + b;
+                if (q.d12 && mu < nu) continue;
+                const int e = (a * NB + b) * NCD + a * ND + b;
+                double v = (double)acc[e] * (fs[mu] * fs[nu] * fs[mu] * fs[nu]);
+                ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
+            }
+        return;
+    }
+    // EPI_JK
+    const int K = ep.K;
+    const double f = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
+    double v[NA * NB * NCD];
+#pragma unroll
+    for (int e = 0; e < NA * NB * NCD; ++e) {
+        const int a = e / (NB * NCD), b = (e / NCD) % NB, c = (e / ND) % NC, d = e % ND;
+        const int mu = q.fn1 + a, nu = q.fn2 + b, la = q.fn3 + c, si = q.fn4 + d;
+        v[e] = mask_pass(ep, mu, nu, la, si) ? (double)acc[e] * (fs[mu] * fs[nu] * fs[la] * fs[si]) : 0.0;
+    }
+    // J
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) s += v[(a * NB + b) * NCD + c * ND + d] * ep.Dj[(q.fn3 + c) * K + q.fn4 + d];
+            atomic_add(&ep.J[(q.fn1 + a) * K + q.fn2 + b], 2.0 * f * s);
+        }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int b = 0; b < NB; ++b) s += v[(a * NB + b) * NCD + c * ND + d] * ep.Dj[(q.fn1 + a) * K + q.fn2 + b];
+            atomic_add(&ep.J[(q.fn3 + c) * K + q.fn4 + d], 2.0 * f * s);
+        }
+    // K (one or two spins)
+    for (int spin = 0; spin < ep.nspin; ++spin) {
+        const double* D = spin == 0 ? ep.Da : ep.Db;
+        double* Kacc = spin == 0 ? ep.Ka : ep.Kb;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                double s = 0.0;   // Kacc_ac += f sum_bd v D_bd
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn2 + b) * K + q.fn4 + d];
+                atomic_add(&Kacc[(q.fn1 + a) * K + q.fn3 + c], f * s);
+            }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                double s = 0.0;   // Kacc_bc += f sum_ad v D_ad
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn1 + a) * K + q.fn4 + d];
+                atomic_add(&Kacc[(q.fn2 + b) * K + q.fn3 + c], f * s);
+            }
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                double s = 0.0;   // Kacc_ad += f sum_bc v D_bc
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn2 + b) * K + q.fn3 + c];
+                atomic_add(&Kacc[(q.fn1 + a) * K + q.fn4 + d], f * s);
+            }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                double s = 0.0;   // Kacc_bd += f sum_ac v D_ac
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn1 + a) * K + q.fn3 + c];
+                atomic_add(&Kacc[(q.fn2 + b) * K + q.fn4 + d], f * s);
+            }
+    }
+}
+
+HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk) {
+    QuartetCtx q;
+    q.fn1 = hb.fn1; q.fn2 = hb.fn2; q.fn3 = hk.fn1; q.fn4 = hk.fn2;
+    q.d12 = hb.diag != 0; q.d34 = hk.diag != 0; q.same = hb.canon == hk.canon;
+    return q;
+}
+
+template <int LA, int LB, int LC, int LD, class Real>
+HFX_HD __forceinline__ void family_a_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
+                                         const EpiArgs& ep) {
+    using F = FamA<LA, LB, LC, LD, Real>;
+    const PairHdr hb = hdr[e.x], hk = hdr[e.y];
+    Real acc[F::NOUT];
+#pragma unroll
+    for (int i = 0; i < F::NOUT; ++i) acc[i] = Real(0);
+    F::quartet(hb, hk, pp, tab, acc);
+    epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real>(make_ctx(hb, hk), ep, acc);
+}
+
+// ------------------------------------------------------------------ family B
+// Shared-memory footprint is sized per class; the ket component pairs (cd)
+// are processed in chunks of CDC columns.
+template <int LA, int LB, int LC, int LD>
+struct FamBShape {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+    static constexpr int NAB = NA * NB, NCD = NC * ND;
+    static constexpr int NHAB = nherm(LAB), NR = nherm(L);
+    static constexpr int BUDGET = 5600;   // doubles for g + acc per chunk
+    static constexpr int CDC0 = BUDGET / (NHAB + NAB);
+    static constexpr int CDC = CDC0 < NCD ? CDC0 : NCD;
+    static constexpr int NCHUNK = (NCD + CDC - 1) / CDC;
+    static constexpr int EB = 3 * (LA + 1) * (LB + 1) * (LAB + 1);
+    static constexpr int EK = 3 * (LC + 1) * (LD + 1) * (LCD + 1);
+};
+
+template <int LA, int LB, int LC, int LD, class Real>
+struct FamBSmem {
+    using S = FamBShape<LA, LB, LC, LD>;
+    Real R[2][S::NR];
+    Real g[S::NHAB * S::CDC];
+    Real acc[S::NAB * S::CDC];
+    Real eb[S::EB];
+    Real ek[S::EK];
+    Real a[S::L + 1];
+    Real rx[3];
+};
+
+// Hermite triple tables for the runtime-indexed loops of family B.
+struct HermTab {
+    unsigned char t[455], u[455], v[455];
+};
+
+template <int LA, int LB, class Real>
+HFX_HD __forceinline__ void build_e_to_smem(Real* dst, const PrimPair<Real>& pp, bool sign) {
+    static_for<0, 3>([&](auto xi) {
+        constexpr int x = decltype(xi)::value;
+        Herm1D<LA, LB, Real> h;
+        h.build(pp.i2p, pp.PA[x], pp.PB[x]);
+        constexpr int S = (LA + 1) * (LB + 1) * (LA + LB + 1);
+#pragma unroll
+        for (int a = 0; a <= LA; ++a)
+#pragma unroll
+            for (int b = 0; b <= LB; ++b)
+#pragma unroll
+                for (int t = 0; t <= LA + LB; ++t)
+                    dst[x * S + (a * (LB + 1) + b) * (LA + LB + 1) + t] = (sign && (t & 1)) ? -h.e[a][b][t] : h.e[a][b][t];
+    });
+}
+
+// One wavefront (or one CPU thread with nlanes = 1) computes one shell quartet.
+// `sync()` must be a wavefront/workgroup barrier on the device.
+template <int LA, int LB, int LC, int LD, class Real, class Sync>
+HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab, const EpiArgs& ep,
+                         FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht, int lane, int nlanes, Sync sync) {
+    using S = FamBShape<LA, LB, LC, LD>;
+    constexpr int SB = (LA + 1) * (LB + 1) * (S::LAB + 1);
+    constexpr int SK = (LC + 1) * (LD + 1) * (S::LCD + 1);
+    const PairHdr hb = hdr[e.x], hk = hdr[e.y];
+    const QuartetCtx q = make_ctx(hb, hk);
+    const double* fs = ep.fscale;
+    const int K = ep.K;
+    const double fdeg = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
+
+    for (int ch = 0; ch < S::NCHUNK; ++ch) {
+        const int cd0 = ch * S::CDC;
+        const int ncd = (S::NCD - cd0) < S::CDC ? (S::NCD - cd0) : S::CDC;
+        for (int w = lane; w < S::NAB * S::CDC; w += nlanes) sm.acc[w] = Real(0);
+        for (int i = 0; i < hb.nprim; ++i) {
+            const PrimPair<Real> b = pp[hb.prim_off + i];
+            sync();
+            if (lane == 0) build_e_to_smem<LA, LB, Real>(sm.eb, b, false);
+            for (int w = lane; w < S::NHAB * S::CDC; w += nlanes) sm.g[w] = Real(0);
+            for (int j = 0; j < hk.nprim; ++j) {
+                const PrimPair<Real> k = pp[hk.prim_off + j];
+                sync();
+                if (lane == 0) build_e_to_smem<LC, LD, Real>(sm.ek, k, true);
+                if (lane == 1 || nlanes == 1) {
+                    Real s = b.p + k.p;
+                    Real inv = Real(1) / s;
+                    Real rho = b.p * k.p * inv;
+                    Real X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
+                    Real T = rho * (X * X + Y * Y + Z * Z);
+                    Real F[S::L + 1];
+                    boys_fm<S::L, Real>(T, tab, F);
+                    Real f = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
+                    Real m2r = Real(-2) * rho;
+                    for (int n = 0; n <= S::L; ++n) {
+                        sm.a[n] = f * F[n];
+                        f *= m2r;
+                    }
+                    sm.rx[0] = X; sm.rx[1] = Y; sm.rx[2] = Z;
+                }
+                sync();
+                // R levels n = L .. 0 : level n holds orders <= L - n, ping-pong buffers
+                int cur = 0;
+                if (lane == 0) sm.R[0][0] = sm.a[S::L];
+                for (int n = S::L - 1; n >= 0; --n) {
+                    sync();
+                    const Real* Rp = sm.R[cur];
+                    Real* Rn = sm.R[cur ^ 1];
+                    const int cnt = nherm(S::L - n);
+                    for (int w = lane; w < cnt; w += nlanes) {
+                        const int t = ht.t[w], u = ht.u[w], v = ht.v[w];
+                        Real val;
+                        if (t > 0) {
+                            val = sm.rx[0] * Rp[hidx(t - 1, u, v)];
+                            if (t > 1) val += Real(t - 1) * Rp[hidx(t - 2, u, v)];
+                        } else if (u > 0) {
+                            val = sm.rx[1] * Rp[hidx(t, u - 1, v)];
+                            if (u > 1) val += Real(u - 1) * Rp[hidx(t, u - 2, v)];
+                        } else if (v > 0) {
+                            val = sm.rx[2] * Rp[hidx(t, u, v - 1)];
+                            if (v > 1) val += Real(v - 1) * Rp[hidx(t, u, v - 2)];
+                        } else {
+                            val = sm.a[n];
+                        }
+                        Rn[w] = val;
+                    }
+                    cur ^= 1;
+                }
+                sync();
+                // stage 1: g[tuv][cd] += sum Eket_cd(tau nu phi) R(t+tau, u+nu, v+phi)
+                const Real* R = sm.R[cur];
+                for (int w = lane; w < S::NHAB * ncd; w += nlanes) {
+                    const int h = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
+                    const int c = cd / S::ND, d = cd % S::ND;
+                    const int t = ht.t[h], u = ht.u[h], v = ht.v[h];
+                    const int cx = cart_x(LC, c), cy = cart_y(LC, c), cz = cart_z(LC, c);
+                    const int dx = cart_x(LD, d), dy = cart_y(LD, d), dz = cart_z(LD, d);
+                    const Real* Ex = sm.ek + (cx * (LD + 1) + dx) * (S::LCD + 1);
+                    const Real* Ey = sm.ek + SK + (cy * (LD + 1) + dy) * (S::LCD + 1);
+                    const Real* Ez = sm.ek + 2 * SK + (cz * (LD + 1) + dz) * (S::LCD + 1);
+                    Real s = Real(0);
+                    for (int ta = 0; ta <= cx + dx; ++ta) {
+                        Real sy = Real(0);
+                        for (int nn = 0; nn <= cy + dy; ++nn) {
+                            Real sz = Real(0);
+                            for (int ph = 0; ph <= cz + dz; ++ph) sz += Ez[ph] * R[hidx(t + ta, u + nn, v + ph)];
+                            sy += Ey[nn] * sz;
+                        }
+                        s += Ex[ta] * sy;
+                    }
+                    sm.g[h * S::CDC + cdl] += s;
+                }
+            }
+            sync();
+            // stage 2: acc[ab][cd] += sum_tuv Ebra_ab(tuv) g[tuv][cd]
+            for (int w = lane; w < S::NAB * ncd; w += nlanes) {
+                const int ab = w / ncd, cdl = w % ncd;
+                const int a = ab / S::NB, bb = ab % S::NB;
+                const int ax = cart_x(LA, a), ay = cart_y(LA, a), az = cart_z(LA, a);
+                const int bx = cart_x(LB, bb), by = cart_y(LB, bb), bz = cart_z(LB, bb);
+                const Real* Ex = sm.eb + (ax * (LB + 1) + bx) * (S::LAB + 1);
+                const Real* Ey = sm.eb + SB + (ay * (LB + 1) + by) * (S::LAB + 1);
+                const Real* Ez = sm.eb + 2 * SB + (az * (LB + 1) + bz) * (S::LAB + 1);
+                Real s = Real(0);
+                for (int t = 0; t <= ax + bx; ++t) {
+                    Real sy = Real(0);
+                    for (int u = 0; u <= ay + by; ++u) {
+                        Real sz = Real(0);
+                        for (int v = 0; v <= az + bz; ++v) sz += Ez[v] * sm.g[hidx(t, u, v) * S::CDC + cdl];
+                        sy += Ey[u] * sz;
+                    }
+                    s += Ex[t] * sy;
+                }
+                sm.acc[ab * S::CDC + cdl] += s;
+            }
+        }
+        sync();
+        // ---------------- epilogue for this chunk (values read from sm.acc)
+        auto val = [&](int ab, int cdl) -> double {
+            const int a = ab / S::NB, bb = ab % S::NB, cd = cd0 + cdl, c = cd / S::ND, d = cd % S::ND;
+            const int mu = q.fn1 + a, nu = q.fn2 + bb, la = q.fn3 + c, si = q.fn4 + d;
+            return (double)sm.acc[ab * S::CDC + cdl] * (fs[mu] * fs[nu] * fs[la] * fs[si]);
+        };
+        if (ep.mode == EPI_UNIQUE) {
+            for (int w = lane; w < S::NAB * ncd; w += nlanes) {
+                const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
+                const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;
+                const int la = q.fn3 + cd / S::ND, si = q.fn4 + cd % S::ND;
+                if (!keep_unique(q, mu, nu, la, si)) continue;
+                const int64_t IJ = pair_of(mu, nu), KL = pair_of(la, si);
+                if (ep.bounds[IJ] * ep.bounds[KL] < ep.threshold) continue;
+                ep.out[quartet_of(IJ, KL)] = val(ab, cdl);
+            }
+        } else if (ep.mode == EPI_DIAG) {
+            for (int w = lane; w < S::NAB; w += nlanes) {
+                const int cd = w;   // (ab|ab): same component pair on both sides
+                if (cd < cd0 || cd >= cd0 + ncd) continue;
+                const int mu = q.fn1 + w / S::NB, nu = q.fn2 + w % S::NB;
+                if (q.d12 && mu < nu) continue;
+                double v = val(w, cd - cd0);
+                ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
+            }
+        } else {
+            auto mv = [&](int ab, int cdl) -> double {
+                const int cd = cd0 + cdl;
+                const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;
+                const int la = q.fn3 + cd / S::ND, si = q.fn4 + cd % S::ND;
+                return mask_pass(ep, mu, nu, la, si) ? val(ab, cdl) : 0.0;
+            };
+            // J_ab over this chunk's cd
+            for (int ab = lane; ab < S::NAB; ab += nlanes) {
+                double s = 0.0;
+                for (int cdl = 0; cdl < ncd; ++cdl) {
+                    const int cd = cd0 + cdl;
+                    s += mv(ab, cdl) * ep.Dj[(q.fn3 + cd / S::ND) * K + q.fn4 + cd % S::ND];
+                }
+                atomic_add(&ep.J[(q.fn1 + ab / S::NB) * K + q.fn2 + ab % S::NB], 2.0 * fdeg * s);
+            }
+            for (int cdl = lane; cdl < ncd; cdl += nlanes) {
+                const int cd = cd0 + cdl;
+                double s = 0.0;
+                for (int ab = 0; ab < S::NAB; ++ab)
+                    s += mv(ab, cdl) * ep.Dj[(q.fn1 + ab / S::NB) * K + q.fn2 + ab % S::NB];
+                atomic_add(&ep.J[(q.fn3 + cd / S::ND) * K + q.fn4 + cd % S::ND], 2.0 * fdeg * s);
+            }
+            for (int spin = 0; spin < ep.nspin; ++spin) {
+                const double* D = spin == 0 ? ep.Da : ep.Db;
+                double* Kacc = spin == 0 ? ep.Ka : ep.Kb;
+                // Kacc_ac (sum b,d) and Kacc_bc (sum a,d): targets indexed by (x, c)
+                for (int w = lane; w < (S::NA + S::NB) * S::NC; w += nlanes) {
+                    const bool isA = w < S::NA * S::NC;
+                    const int x = isA ? w / S::NC : (w - S::NA * S::NC) / S::NC;
+                    const int c = w % S::NC;
+                    double s = 0.0;
+                    bool any = false;
+                    for (int d = 0; d < S::ND; ++d) {
+                        const int cd = c * S::ND + d;
+                        if (cd < cd0 || cd >= cd0 + ncd) continue;
+                        any = true;
+                        const int cdl = cd - cd0;
+                        if (isA) {
+                            for (int bb = 0; bb < S::NB; ++bb)
+                                s += mv(x * S::NB + bb, cdl) * D[(q.fn2 + bb) * K + q.fn4 + d];
+                        } else {
+                            for (int a = 0; a < S::NA; ++a)
+                                s += mv(a * S::NB + x, cdl) * D[(q.fn1 + a) * K + q.fn4 + d];
+                        }
+                    }
+                    if (any) {
+                        const int row = isA ? q.fn1 + x : q.fn2 + x;
+                        atomic_add(&Kacc[row * K + q.fn3 + c], fdeg * s);
+                    }
+                }
+                // Kacc_ad (sum b,c) and Kacc_bd (sum a,c): targets indexed by (x, d)
+                for (int w = lane; w < (S::NA + S::NB) * S::ND; w += nlanes) {
+                    const bool isA = w < S::NA * S::ND;
+                    const int x = isA ? w / S::ND : (w - S::NA * S::ND) / S::ND;
+                    const int d = w % S::ND;
+                    double s = 0.0;
+                    bool any = false;
+                    for (int c = 0; c < S::NC; ++c) {
+                        const int cd = c * S::ND + d;
+                        if (cd < cd0 || cd >= cd0 + ncd) continue;
+                        any = true;
+                        const int cdl = cd - cd0;
+                        if (isA) {
+                            for (int bb = 0; bb < S::NB; ++bb)
+                                s += mv(x * S::NB + bb, cdl) * D[(q.fn2 + bb) * K + q.fn3 + c];
+                        } else {
+                            for (int a = 0; a < S::NA; ++a)
+                                s += mv(a * S::NB + x, cdl) * D[(q.fn1 + a) * K + q.fn3 + c];
+                        }
+                    }
+                    if (any) {
+                        const int row = isA ? q.fn1 + x : q.fn2 + x;
+                        atomic_add(&Kacc[row * K + q.fn4 + d], fdeg * s);
+                    }
+                }
+            }
+        }
+    }
+    sync();
+}
+
+}  // namespace hfx

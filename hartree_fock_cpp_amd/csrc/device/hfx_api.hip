@@ -1,0 +1,502 @@
+// hfx_api.hip -- C ABI (include/hfx.h) over the gfx950 kernels.
+//
+// Owns the device-resident tables of a basis (hfx_basis) and of one SCF job's
+// Fock builds (hfx_fock).  Every HIP call is checked; errors come back as
+// status codes with a thread-local message, never as exit().
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/hfx.h"
+#include "hfx_launch.hpp"
+#include "plan.hpp"
+
+using namespace hfx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return fail(_e == hipErrorOutOfMemory ? HFX_ERR_NOMEM : HFX_ERR_DEVICE,           \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+    } while (0)
+
+// Process-wide device constants: Boys tables (FP64 and FP32) and the Hermite
+// index table in constant memory.  Built on first use.
+struct DeviceConsts {
+    std::once_flag once;
+    int status = 0;
+    std::string err;
+    double* boys_d = nullptr;
+    float* boys_f = nullptr;
+};
+DeviceConsts g_consts;
+
+int ensure_consts() {
+    std::call_once(g_consts.once, []() {
+        std::vector<double> tab;
+        boys_table(tab);
+        std::vector<float> tabf(tab.begin(), tab.end());
+        hipError_t e = hipMalloc(&g_consts.boys_d, tab.size() * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&g_consts.boys_f, tabf.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(g_consts.boys_d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(g_consts.boys_f, tabf.data(), tabf.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = upload_herm_tab(herm_tab());
+        if (e != hipSuccess) {
+            g_consts.status = HFX_ERR_DEVICE;
+            g_consts.err = std::string("device constant setup: ") + hipGetErrorString(e);
+        }
+    });
+    if (g_consts.status) return fail(g_consts.status, g_consts.err);
+    return 0;
+}
+
+template <class T>
+int dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return 0;
+    hipError_t e = hipMalloc(p, n * sizeof(T));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? HFX_ERR_NOMEM : HFX_ERR_DEVICE,
+                    std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct hfx_basis {
+    BasisPlan plan;
+    PairHdr* d_hdr = nullptr;
+    PrimPair<double>* d_ppd = nullptr;
+    PrimPair<float>* d_ppf = nullptr;
+    double* d_fscale = nullptr;
+    double* d_bounds = nullptr;      // function-pair Schwarz bounds (P)
+    int2* d_fnpairs = nullptr;       // function pair id -> (i, j)
+    bool bounds_ready = false;
+    std::vector<double> bounds;      // host copy
+    std::vector<double> Qsh;         // shell-pair maxima
+    hipStream_t stream = nullptr;
+};
+
+struct hfx_fock {
+    hfx_basis* b = nullptr;
+    hfx_fock_opts opts{};
+    int mode = HFX_MODE_STORED;
+    size_t bytes = 0;
+    size_t nq = 0;
+    double* d_eri = nullptr;                 // stored mode
+    uint2* d_lists = nullptr;                // direct mode: all lists back to back
+    size_t off[NQCLASS][2] = {};
+    int cnt[NQCLASS][2] = {};
+    double *d_Dj = nullptr, *d_Da = nullptr, *d_Db = nullptr;
+    double *d_J = nullptr, *d_Ka = nullptr, *d_Kb = nullptr;
+};
+
+namespace {
+
+size_t n_unique(int K) {
+    const size_t P = (size_t)K * (K + 1) / 2;
+    return P * (P + 1) / 2;
+}
+
+EpiArgs base_epi(const hfx_basis* b) {
+    EpiArgs ep{};
+    ep.K = b->plan.K;
+    ep.bounds = b->d_bounds;
+    ep.fscale = b->d_fscale;
+    ep.nspin = 1;
+    return ep;
+}
+
+int launch_lists(const hfx_basis* b, const uint2* d_lists, const size_t (*off)[2], const int (*cnt)[2],
+                 const EpiArgs& ep, hipStream_t st) {
+    for (int qc = 0; qc < NQCLASS; ++qc)
+        for (int p = 0; p < 2; ++p) {
+            if (cnt[qc][p] == 0) continue;
+            ClassLaunch L;
+            L.list = d_lists + off[qc][p];
+            L.n = cnt[qc][p];
+            L.hdr = b->d_hdr;
+            L.prim = p ? (const void*)b->d_ppf : (const void*)b->d_ppd;
+            L.boys = p ? (const void*)g_consts.boys_f : (const void*)g_consts.boys_d;
+            L.ep = ep;
+            HIPCHK(launch_eri_class(qc, p == 1, L, st));
+        }
+    return 0;
+}
+
+// Uploads quartet lists into one device buffer; fills offsets / counts.
+int upload_lists(const QuartetLists& ql, uint2** d_lists, size_t (*off)[2], int (*cnt)[2]) {
+    std::vector<uint2> all;
+    all.reserve(ql.total());
+    for (int qc = 0; qc < NQCLASS; ++qc)
+        for (int p = 0; p < 2; ++p) {
+            off[qc][p] = all.size();
+            cnt[qc][p] = (int)ql.list[qc][p].size();
+            all.insert(all.end(), ql.list[qc][p].begin(), ql.list[qc][p].end());
+        }
+    if (int rc = dalloc(d_lists, all.size())) return rc;
+    if (!all.empty()) HIPCHK(hipMemcpy(*d_lists, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int compute_bounds(hfx_basis* b) {
+    if (b->bounds_ready) return 0;
+    const int P = b->plan.K * (b->plan.K + 1) / 2;
+    QuartetLists ql;
+    std::vector<uint2> diag[NQCLASS];
+    plan_diag(b->plan, diag);
+    for (int c = 0; c < NQCLASS; ++c) ql.list[c][0] = diag[c];
+    uint2* d_lists = nullptr;
+    size_t off[NQCLASS][2];
+    int cnt[NQCLASS][2];
+    if (int rc = upload_lists(ql, &d_lists, off, cnt)) return rc;
+    HIPCHK(launch_zero(b->d_bounds, P, b->stream));
+    EpiArgs ep = base_epi(b);
+    ep.mode = EPI_DIAG;
+    ep.out = b->d_bounds;
+    int rc = launch_lists(b, d_lists, off, cnt, ep, b->stream);
+    if (rc == 0) {
+        b->bounds.resize(P);
+        hipError_t e = hipMemcpyAsync(b->bounds.data(), b->d_bounds, P * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("schwarz bounds: ") + hipGetErrorString(e));
+    }
+    dfree(d_lists);
+    if (rc) return rc;
+    shell_pair_bounds(b->plan, b->bounds.data(), b->Qsh);
+    b->bounds_ready = true;
+    return 0;
+}
+
+int eri_unique_into(hfx_basis* b, double threshold, double* d_out, hipStream_t st) {
+    if (int rc = compute_bounds(b)) return rc;
+    QuartetLists ql;
+    plan_quartets(b->plan, b->Qsh, threshold, false, 0.0, ql);
+    uint2* d_lists = nullptr;
+    size_t off[NQCLASS][2];
+    int cnt[NQCLASS][2];
+    if (int rc = upload_lists(ql, &d_lists, off, cnt)) return rc;
+    int rc = 0;
+    hipError_t e = launch_zero(d_out, n_unique(b->plan.K), st);
+    if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("zero: ") + hipGetErrorString(e));
+    if (rc == 0) {
+        EpiArgs ep = base_epi(b);
+        ep.mode = EPI_UNIQUE;
+        ep.threshold = threshold;
+        ep.out = d_out;
+        rc = launch_lists(b, d_lists, off, cnt, ep, st);
+    }
+    if (rc == 0) {
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("eri kernels: ") + hipGetErrorString(e));
+    }
+    dfree(d_lists);
+    return rc;
+}
+
+// J/K from a device unique array into device accumulators, then symmetrised.
+int jk_from_unique(hfx_basis* b, const double* d_eri, const double* d_Dj, const double* d_Da, const double* d_Db,
+                   double* d_J, double* d_Ka, double* d_Kb, int nspin, hipStream_t st) {
+    const int K = b->plan.K;
+    const size_t K2 = (size_t)K * K;
+    HIPCHK(launch_zero(d_J, K2, st));
+    HIPCHK(launch_zero(d_Ka, K2, st));
+    if (nspin == 2) HIPCHK(launch_zero(d_Kb, K2, st));
+    HIPCHK(launch_jk_unique(d_eri, b->d_fnpairs, K, nspin, d_Dj, d_Da, d_Db, d_J, d_Ka, d_Kb, st));
+    HIPCHK(launch_symmetrize(d_J, K, st));
+    HIPCHK(launch_symmetrize(d_Ka, K, st));
+    if (nspin == 2) HIPCHK(launch_symmetrize(d_Kb, K, st));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hfx_last_error(void) { return g_err.c_str(); }
+
+int hfx_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, const double* alpha,
+                     const double* coeff, const double* norm, hfx_basis** out) {
+    if (!out || !xyz || !lmn || !poff || !alpha || !coeff || !norm) return fail(HFX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (int rc = ensure_consts()) return rc;
+    hfx_basis* b = new hfx_basis();
+    std::string err;
+    int rc = build_plan(K, xyz, lmn, poff, alpha, coeff, norm, b->plan, err);
+    if (rc) {
+        delete b;
+        return fail(rc == -4 ? HFX_ERR_UNSUPPORTED : HFX_ERR_INVALID, err);
+    }
+    const BasisPlan& p = b->plan;
+    const int P = K * (K + 1) / 2;
+    rc = dalloc(&b->d_hdr, p.hdr.size());
+    if (!rc) rc = dalloc(&b->d_ppd, std::max<size_t>(1, p.ppd.size()));
+    if (!rc) rc = dalloc(&b->d_ppf, std::max<size_t>(1, p.ppf.size()));
+    if (!rc) rc = dalloc(&b->d_fscale, (size_t)K);
+    if (!rc) rc = dalloc(&b->d_bounds, (size_t)P);
+    if (!rc) rc = dalloc(&b->d_fnpairs, (size_t)P);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (!rc && e == hipSuccess) e = hipMemcpy(b->d_hdr, p.hdr.data(), p.hdr.size() * sizeof(PairHdr), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess && !p.ppd.empty())
+        e = hipMemcpy(b->d_ppd, p.ppd.data(), p.ppd.size() * sizeof(PrimPair<double>), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess && !p.ppf.empty())
+        e = hipMemcpy(b->d_ppf, p.ppf.data(), p.ppf.size() * sizeof(PrimPair<float>), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemcpy(b->d_fscale, p.fscale.data(), K * sizeof(double), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemcpy(b->d_fnpairs, p.fnpairs.data(), P * sizeof(int2), hipMemcpyHostToDevice);
+    if (!rc && e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("basis upload: ") + hipGetErrorString(e));
+    if (rc) {
+        hfx_basis_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+void hfx_basis_destroy(hfx_basis* b) {
+    if (!b) return;
+    dfree(b->d_hdr);
+    dfree(b->d_ppd);
+    dfree(b->d_ppf);
+    dfree(b->d_fscale);
+    dfree(b->d_bounds);
+    dfree(b->d_fnpairs);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int hfx_basis_info(const hfx_basis* b, int* nshell, int* npair, int* nprimpair) {
+    if (!b) return fail(HFX_ERR_INVALID, "null basis");
+    if (nshell) *nshell = b->plan.nshell;
+    if (npair) *npair = b->plan.npair;
+    if (nprimpair) *nprimpair = (int)b->plan.ppd.size();
+    return 0;
+}
+
+int hfx_schwarz_bounds(hfx_basis* b, double* bounds) {
+    if (!b || !bounds) return fail(HFX_ERR_INVALID, "null argument");
+    if (int rc = compute_bounds(b)) return rc;
+    std::memcpy(bounds, b->bounds.data(), b->bounds.size() * sizeof(double));
+    return 0;
+}
+
+int hfx_eri_unique_device(hfx_basis* b, double threshold, double* d_out, size_t Q, void* stream) {
+    if (!b || !d_out) return fail(HFX_ERR_INVALID, "null argument");
+    if (Q != n_unique(b->plan.K)) return fail(HFX_ERR_INVALID, "Q must equal P(P+1)/2 with P = K(K+1)/2");
+    return eri_unique_into(b, threshold, d_out, stream ? (hipStream_t)stream : b->stream);
+}
+
+int hfx_eri_unique(hfx_basis* b, double threshold, double* out, size_t Q) {
+    if (!b || !out) return fail(HFX_ERR_INVALID, "null argument");
+    if (Q != n_unique(b->plan.K)) return fail(HFX_ERR_INVALID, "Q must equal P(P+1)/2 with P = K(K+1)/2");
+    double* d = nullptr;
+    if (int rc = dalloc(&d, Q)) return rc;
+    int rc = eri_unique_into(b, threshold, d, b->stream);
+    if (rc == 0) {
+        hipError_t e = hipMemcpy(out, d, Q * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("D2H: ") + hipGetErrorString(e));
+    }
+    dfree(d);
+    return rc;
+}
+
+int hfx_coulomb_exchange(hfx_basis* b, const double* d_eri, const double* Pa, const double* Pb, double* J,
+                         double* Ka, double* Kb) {
+    if (!b || !d_eri || !Pa) return fail(HFX_ERR_INVALID, "null argument");
+    const int K = b->plan.K;
+    const size_t K2 = (size_t)K * K;
+    const int nspin = Pb ? 2 : 1;
+    std::vector<double> Dj(Pa, Pa + K2);
+    if (Pb)
+        for (size_t i = 0; i < K2; ++i) Dj[i] += Pb[i];
+    double* buf = nullptr;
+    if (int rc = dalloc(&buf, 6 * K2)) return rc;
+    double *dDj = buf, *dDa = buf + K2, *dDb = buf + 2 * K2, *dJ = buf + 3 * K2, *dKa = buf + 4 * K2, *dKb = buf + 5 * K2;
+    int rc = 0;
+    hipError_t e = hipMemcpy(dDj, Dj.data(), K2 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dDa, Pa, K2 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && Pb) e = hipMemcpy(dDb, Pb, K2 * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("H2D: ") + hipGetErrorString(e));
+    if (!rc) rc = jk_from_unique(b, d_eri, dDj, dDa, dDb, dJ, dKa, dKb, nspin, b->stream);
+    if (!rc) {
+        e = hipStreamSynchronize(b->stream);
+        if (e == hipSuccess && J) e = hipMemcpy(J, dJ, K2 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && Ka) e = hipMemcpy(Ka, dKa, K2 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && Kb && Pb) e = hipMemcpy(Kb, dKb, K2 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("J/K: ") + hipGetErrorString(e));
+    }
+    dfree(buf);
+    return rc;
+}
+
+int hfx_repulsion_matrix(hfx_basis* b, const double* d_eri, const double* P, double* G) {
+    if (!b || !G) return fail(HFX_ERR_INVALID, "null argument");
+    const int K = b->plan.K;
+    std::vector<double> J((size_t)K * K), Kx((size_t)K * K);
+    if (int rc = hfx_coulomb_exchange(b, d_eri, P, nullptr, J.data(), Kx.data(), nullptr)) return rc;
+    for (size_t i = 0; i < J.size(); ++i) G[i] = J[i] - 0.5 * Kx[i];
+    return 0;
+}
+
+int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
+    if (!b || !opts || !out) return fail(HFX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (opts->mode < 0 || opts->mode > 2 || opts->precision < 0 || opts->precision > 1)
+        return fail(HFX_ERR_INVALID, "invalid mode or precision");
+    if (int rc = compute_bounds(b)) return rc;
+    const int K = b->plan.K;
+    const size_t K2 = (size_t)K * K;
+    const size_t Q = n_unique(K);
+    hfx_fock* f = new hfx_fock();
+    f->b = b;
+    f->opts = *opts;
+    if (f->opts.mixed_split <= 0.0) f->opts.mixed_split = 1e-5;
+    const size_t mats = 6 * K2 * sizeof(double);
+    int mode = opts->mode;
+    if (opts->precision == HFX_PREC_MIXED) mode = HFX_MODE_DIRECT;   // the mixed form is a direct build
+    QuartetLists ql;
+    plan_quartets(b->plan, b->Qsh, opts->threshold, opts->precision == HFX_PREC_MIXED, f->opts.mixed_split, ql);
+    f->nq = ql.total();
+    const size_t stored_bytes = Q * sizeof(double) + mats;
+    const size_t direct_bytes = f->nq * sizeof(uint2) + mats;
+    if (mode == HFX_MODE_AUTO) mode = (opts->hbm_budget == 0 || stored_bytes <= opts->hbm_budget) ? HFX_MODE_STORED
+                                                                                                 : HFX_MODE_DIRECT;
+    f->mode = mode;
+    f->bytes = mode == HFX_MODE_STORED ? stored_bytes : direct_bytes;
+    if (opts->hbm_budget && f->bytes > opts->hbm_budget) {
+        delete f;
+        return fail(HFX_ERR_NOMEM, "job does not fit its HBM budget");
+    }
+    double* buf = nullptr;
+    int rc = dalloc(&buf, 6 * K2);
+    if (!rc) {
+        f->d_Dj = buf;
+        f->d_Da = buf + K2;
+        f->d_Db = buf + 2 * K2;
+        f->d_J = buf + 3 * K2;
+        f->d_Ka = buf + 4 * K2;
+        f->d_Kb = buf + 5 * K2;
+    }
+    if (!rc) {
+        for (int qc = 0; qc < NQCLASS; ++qc)
+            for (int p = 0; p < 2; ++p) f->cnt[qc][p] = (int)ql.list[qc][p].size();
+        if (mode == HFX_MODE_STORED) {
+            rc = dalloc(&f->d_eri, Q);
+            if (!rc) rc = eri_unique_into(b, opts->threshold, f->d_eri, b->stream);
+        } else {
+            rc = upload_lists(ql, &f->d_lists, f->off, f->cnt);
+        }
+    }
+    if (rc) {
+        hfx_fock_destroy(f);
+        return rc;
+    }
+    *out = f;
+    return 0;
+}
+
+void hfx_fock_destroy(hfx_fock* f) {
+    if (!f) return;
+    dfree(f->d_eri);
+    dfree(f->d_lists);
+    dfree(f->d_Dj);   // one allocation for all six matrices
+    delete f;
+}
+
+int hfx_fock_info(const hfx_fock* f, int* mode, size_t* device_bytes, size_t* n_quartets) {
+    if (!f) return fail(HFX_ERR_INVALID, "null fock");
+    if (mode) *mode = f->mode;
+    if (device_bytes) *device_bytes = f->bytes;
+    if (n_quartets) *n_quartets = f->nq;
+    return 0;
+}
+
+int hfx_fock_class_counts(const hfx_fock* f, long long* n64, long long* n32) {
+    if (!f) return fail(HFX_ERR_INVALID, "null fock");
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        if (n64) n64[qc] = f->cnt[qc][0];
+        if (n32) n32[qc] = f->cnt[qc][1];
+    }
+    return 0;
+}
+
+int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka,
+                          double* d_Kb, void* stream) {
+    if (!f || !d_Pa || !d_J || !d_Ka) return fail(HFX_ERR_INVALID, "null argument");
+    hfx_basis* b = f->b;
+    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    const int K = b->plan.K;
+    const size_t K2 = (size_t)K * K;
+    const int nspin = d_Pb ? 2 : 1;
+    if (nspin == 2 && !d_Kb) return fail(HFX_ERR_INVALID, "UHF build needs Kb");
+    // Dj = Pa (+ Pb) for the Coulomb part
+    const double* Dj = d_Pa;
+    if (nspin == 2) {
+        HIPCHK(hipMemcpyAsync(f->d_Dj, d_Pa, K2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIPCHK(launch_axpy(f->d_Dj, d_Pb, K2, st));
+        Dj = f->d_Dj;
+    }
+    if (f->mode == HFX_MODE_STORED) return jk_from_unique(b, f->d_eri, Dj, d_Pa, d_Pb, d_J, d_Ka, d_Kb, nspin, st);
+    HIPCHK(launch_zero(d_J, K2, st));
+    HIPCHK(launch_zero(d_Ka, K2, st));
+    if (nspin == 2) HIPCHK(launch_zero(d_Kb, K2, st));
+    EpiArgs ep = base_epi(b);
+    ep.mode = EPI_JK;
+    ep.threshold = f->opts.threshold;
+    ep.nspin = nspin;
+    ep.Dj = Dj;
+    ep.Da = d_Pa;
+    ep.Db = d_Pb;
+    ep.J = d_J;
+    ep.Ka = d_Ka;
+    ep.Kb = d_Kb;
+    if (int rc = launch_lists(b, f->d_lists, f->off, f->cnt, ep, st)) return rc;
+    HIPCHK(launch_symmetrize(d_J, K, st));
+    HIPCHK(launch_symmetrize(d_Ka, K, st));
+    if (nspin == 2) HIPCHK(launch_symmetrize(d_Kb, K, st));
+    return 0;
+}
+
+int hfx_fock_build(hfx_fock* f, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb) {
+    if (!f || !Pa) return fail(HFX_ERR_INVALID, "null argument");
+    hfx_basis* b = f->b;
+    const int K = b->plan.K;
+    const size_t K2 = (size_t)K * K;
+    HIPCHK(hipMemcpyAsync(f->d_Da, Pa, K2 * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    if (Pb) HIPCHK(hipMemcpyAsync(f->d_Db, Pb, K2 * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    if (int rc = hfx_fock_build_device(f, f->d_Da, Pb ? f->d_Db : nullptr, f->d_J, f->d_Ka, f->d_Kb, b->stream))
+        return rc;
+    if (J) HIPCHK(hipMemcpyAsync(J, f->d_J, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (Ka) HIPCHK(hipMemcpyAsync(Ka, f->d_Ka, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (Kb && Pb) HIPCHK(hipMemcpyAsync(Kb, f->d_Kb, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,208 @@
+// hfx_kernels.hip -- __global__ entry points for gfx950 and their launch table.
+//
+//  * k_fam_a<class, Real>   : one lane per shell quartet (low-L classes)
+//  * k_fam_b<class, Real>   : one 64-lane wavefront per shell quartet (the rest)
+//  * k_jk_unique            : J/K from the stored 8-fold unique ERI array
+//  * k_fock_finish          : J = Jacc + Jacc^T, K = Kacc + Kacc^T, G / F assembly
+//
+// Every ERI kernel reads a class-homogeneous list of (bra pair, ket pair)
+// ids, so one launch covers one angular class and one precision.
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "eri_body.hpp"
+#include "eri_classes.hpp"
+#include "hfx_launch.hpp"
+
+namespace hfx {
+
+__constant__ HermTab c_herm_tab;
+
+template <int LA, int LB, int LC, int LD, class Real>
+__global__ __launch_bounds__(128) void k_fam_a(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
+                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
+                                                EpiArgs ep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    family_a_one<LA, LB, LC, LD, Real>(list[i], hdr, pp, tab, ep);
+}
+
+template <int LA, int LB, int LC, int LD, class Real>
+__global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
+                                               const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
+                                               EpiArgs ep) {
+    __shared__ FamBSmem<LA, LB, LC, LD, Real> sm;
+    auto sync = []() { __syncthreads(); };
+    for (int q = blockIdx.x; q < n; q += gridDim.x)
+        family_b_one<LA, LB, LC, LD, Real>(list[q], hdr, pp, tab, ep, sm, c_herm_tab, (int)threadIdx.x, 64, sync);
+}
+
+template <int QC, class Real>
+static hipError_t launch_class(const ClassLaunch& L, hipStream_t st) {
+    using C = QClass<QC>;
+    if (L.n <= 0) return hipSuccess;
+    const PrimPair<Real>* pp = static_cast<const PrimPair<Real>*>(L.prim);
+    const Real* tab = static_cast<const Real*>(L.boys);
+    if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
+        const int block = 128;
+        const int grid = (L.n + block - 1) / block;
+        k_fam_a<C::LA, C::LB, C::LC, C::LD, Real><<<grid, block, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep);
+    } else {
+        const int grid = L.n < 65536 ? L.n : 65536;
+        k_fam_b<C::LA, C::LB, C::LC, C::LD, Real><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep);
+    }
+    return hipGetLastError();
+}
+
+template <class Real, int... QC>
+static constexpr LaunchFn make_table(std::integer_sequence<int, QC...>, int qc) {
+    constexpr LaunchFn t[] = {&launch_class<QC, Real>...};
+    return t[qc];
+}
+
+hipError_t launch_eri_class(int qc, bool fp32, const ClassLaunch& L, hipStream_t st) {
+    if (qc < 0 || qc >= NQCLASS) return hipErrorInvalidValue;
+    LaunchFn f = fp32 ? make_table<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)
+                      : make_table<double>(std::make_integer_sequence<int, NQCLASS>{}, qc);
+    return f(L, st);
+}
+
+bool class_uses_family_a(int qc) {
+    int pb = qc_bra(qc), pk = qc_ket(qc);
+    return use_family_a(pc_la(pb), pc_lb(pb), pc_la(pk), pc_lb(pk));
+}
+
+hipError_t upload_herm_tab(const HermTab& h) { return hipMemcpyToSymbol(HIP_SYMBOL(c_herm_tab), &h, sizeof(HermTab)); }
+
+// ------------------------------------------------------------------ stored-ERI J/K
+// One workgroup per function-pair row IJ = pair(i, j) of the unique array
+// (rows longest-first).  Row elements (ij|kl), KL <= IJ, are read once,
+// coalesced.  With f the function-level degeneracy factor:
+//   Jacc_ij += 2f v Dj_kl   (block reduction, one atomic per row)
+//   Jacc_kl += 2f v Dj_ij   (one atomic per element)
+//   Kacc_ik += f v D_jl, Kacc_il += f v D_jk  (row i, LDS accumulator)
+//   Kacc_jk += f v D_il, Kacc_jl += f v D_ik  (row j, LDS accumulator)
+constexpr int JK_BLOCK = 256;
+constexpr int JK_MAXK = 900;   // 2 spins x 4 rows x K doubles must fit 64 KiB of LDS
+
+__global__ __launch_bounds__(JK_BLOCK) void k_jk_unique(const double* __restrict__ eri, const int2* __restrict__ pairs,
+                                                         int K, int npair, int nspin, const double* __restrict__ Dj,
+                                                         const double* __restrict__ Da, const double* __restrict__ Db,
+                                                         double* J, double* Ka, double* Kb) {
+    extern __shared__ double lds[];
+    // layout: [spin][ Krow_i[K], Krow_j[K], D_i[K], D_j[K] ], then red[JK_BLOCK/64]
+    const int IJ = npair - 1 - (int)blockIdx.x;
+    const int2 ij = pairs[IJ];
+    const int i = ij.x, j = ij.y;
+    const int tid = threadIdx.x;
+    for (int s = 0; s < nspin; ++s) {
+        double* base = lds + s * 4 * K;
+        const double* D = s == 0 ? Da : Db;
+        for (int c = tid; c < K; c += JK_BLOCK) {
+            base[c] = 0.0;
+            base[K + c] = 0.0;
+            base[2 * K + c] = D[i * K + c];
+            base[3 * K + c] = D[j * K + c];
+        }
+    }
+    __syncthreads();
+    const double dij = Dj[i * K + j];
+    const double fij = (i == j) ? 0.5 : 1.0;
+    const size_t rowbase = (size_t)IJ * (IJ + 1) / 2;
+    double srow = 0.0;
+    for (int KL = tid; KL <= IJ; KL += JK_BLOCK) {
+        const double v = eri[rowbase + KL];
+        if (v == 0.0) continue;
+        const int2 kl = pairs[KL];
+        const int k = kl.x, l = kl.y;
+        const double f = fij * (k == l ? 0.5 : 1.0) * (KL == IJ ? 0.5 : 1.0);
+        const double fv = f * v;
+        srow += 2.0 * fv * Dj[k * K + l];
+        unsafeAtomicAdd(&J[k * K + l], 2.0 * fv * dij);
+        for (int s = 0; s < nspin; ++s) {
+            double* base = lds + s * 4 * K;
+            const double* Di = base + 2 * K;
+            const double* Dj_ = base + 3 * K;
+            atomicAdd(&base[k], fv * Dj_[l]);          // Kacc_ik += f v D_jl
+            atomicAdd(&base[l], fv * Dj_[k]);          // Kacc_il += f v D_jk
+            atomicAdd(&base[K + k], fv * Di[l]);       // Kacc_jk += f v D_il
+            atomicAdd(&base[K + l], fv * Di[k]);       // Kacc_jl += f v D_ik
+        }
+    }
+    // block reduction of the row sum
+    for (int off = 32; off > 0; off >>= 1) srow += __shfl_down(srow, off, 64);
+    double* red = lds + nspin * 4 * K;
+    if ((tid & 63) == 0) red[tid >> 6] = srow;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < JK_BLOCK / 64; ++w) s += red[w];
+        unsafeAtomicAdd(&J[i * K + j], s);
+    }
+    for (int s = 0; s < nspin; ++s) {
+        double* base = lds + s * 4 * K;
+        double* Kx = s == 0 ? Ka : Kb;
+        for (int c = tid; c < K; c += JK_BLOCK) {
+            if (base[c] != 0.0) unsafeAtomicAdd(&Kx[i * K + c], base[c]);
+            if (base[K + c] != 0.0) unsafeAtomicAdd(&Kx[j * K + c], base[K + c]);
+        }
+    }
+}
+
+hipError_t launch_jk_unique(const double* eri, const int2* pairs, int K, int nspin, const double* Dj, const double* Da,
+                            const double* Db, double* J, double* Ka, double* Kb, hipStream_t st) {
+    if (K > JK_MAXK) return hipErrorInvalidValue;
+    const int npair = K * (K + 1) / 2;
+    const size_t lds = sizeof(double) * (size_t)(nspin * 4 * K + JK_BLOCK / 64);
+    k_jk_unique<<<npair, JK_BLOCK, lds, st>>>(eri, pairs, K, npair, nspin, Dj, Da, Db, J, Ka, Kb);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ finish
+// J <- Jacc + Jacc^T ; Kx <- Kacc + Kacc^T (in place, one thread per i <= j pair)
+__global__ void k_symmetrize(double* M, int K) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = (int64_t)K * (K + 1) / 2;
+    if (p >= np) return;
+    int i = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((int64_t)i * (i + 1) / 2 > p) --i;
+    while ((int64_t)(i + 1) * (i + 2) / 2 <= p) ++i;
+    const int j = (int)(p - (int64_t)i * (i + 1) / 2);
+    const double s = M[i * K + j] + M[j * K + i];
+    M[i * K + j] = s;
+    M[j * K + i] = s;
+}
+
+hipError_t launch_symmetrize(double* M, int K, hipStream_t st) {
+    const int64_t np = (int64_t)K * (K + 1) / 2;
+    const int block = 256;
+    k_symmetrize<<<(unsigned)((np + block - 1) / block), block, 0, st>>>(M, K);
+    return hipGetLastError();
+}
+
+__global__ void k_zero(double* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+__global__ void k_axpy(double* y, const double* x, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += x[i];
+}
+
+hipError_t launch_axpy(double* y, const double* x, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_axpy<<<(unsigned)blocks, 256, 0, st>>>(y, x, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(double* p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_zero<<<(unsigned)blocks, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace hfx

@@ -1,0 +1,29 @@
+// hfx_launch.hpp -- host-side launch interface of hfx_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "eri_body.hpp"
+
+namespace hfx {
+
+struct ClassLaunch {
+    const uint2* list;     // device (bra pair, ket pair) ids of one class
+    int n;
+    const PairHdr* hdr;    // device pair headers (indexed by canonical pair id)
+    const void* prim;      // device PrimPair<Real> array
+    const void* boys;      // device Boys table (Real)
+    EpiArgs ep;
+};
+
+using LaunchFn = hipError_t (*)(const ClassLaunch&, hipStream_t);
+
+hipError_t launch_eri_class(int qc, bool fp32, const ClassLaunch& L, hipStream_t st);
+bool class_uses_family_a(int qc);
+hipError_t upload_herm_tab(const HermTab& h);
+hipError_t launch_jk_unique(const double* eri, const int2* pairs, int K, int nspin, const double* Dj, const double* Da,
+                            const double* Db, double* J, double* Ka, double* Kb, hipStream_t st);
+hipError_t launch_symmetrize(double* M, int K, hipStream_t st);
+hipError_t launch_zero(double* p, size_t n, hipStream_t st);
+hipError_t launch_axpy(double* y, const double* x, size_t n, hipStream_t st);   // y += x
+
+}  // namespace hfx

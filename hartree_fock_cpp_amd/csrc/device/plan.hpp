@@ -1,0 +1,270 @@
+// plan.hpp -- host-side preparation of the device tables (shells, shell pairs,
+// primitive pairs, quartet lists, Boys table).  Pure host C++, shared by the
+// HIP library and the CPU emulation build used by the tests.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "eri_body.hpp"
+#include "eri_classes.hpp"
+
+namespace hfx {
+
+struct ShellH {
+    double c[3];
+    int L;
+    int nprim;
+    int poff;   // into BasisPlan::alpha / w
+    int fn0;    // first basis function
+};
+
+struct BasisPlan {
+    int K = 0;
+    int nshell = 0;
+    int npair = 0;
+    std::vector<ShellH> shells;
+    std::vector<double> alpha, w;       // shell primitives: exponent, coeff * radial norm
+    std::vector<double> fscale;         // per function: component norm / radial norm
+    std::vector<PairHdr> hdr;           // indexed by canonical pair id
+    std::vector<int> pclass;            // pair class per canonical pair id
+    std::vector<PrimPair<double>> ppd;
+    std::vector<PrimPair<float>> ppf;
+    std::vector<int2> fnpairs;          // function pair id -> (i, j), i >= j
+};
+
+// Drop primitive pairs whose Gaussian-product prefactor exp(-mu |AB|^2) is
+// below e^-60 (~9e-27): their contribution to any integral is < 1e-22.
+constexpr double PRIM_PAIR_CUT = 60.0;
+
+inline int build_plan(int K, const double* xyz, const int* lmn, const int* poff, const double* alpha,
+                      const double* coeff, const double* norm, BasisPlan& bp, std::string& err) {
+    bp = BasisPlan();
+    bp.K = K;
+    bp.fscale.assign(K, 1.0);
+    if (K <= 0) {
+        err = "basis has no functions";
+        return -1;
+    }
+    int mu = 0;
+    while (mu < K) {
+        const int L = lmn[3 * mu] + lmn[3 * mu + 1] + lmn[3 * mu + 2];
+        if (L > MAX_L) {
+            err = "angular momentum above f (L > 3) is not supported by the GPU path";
+            return -4;
+        }
+        const int n = ncart(L);
+        const int np = poff[mu + 1] - poff[mu];
+        if (mu + n > K || np <= 0) {
+            err = "basis functions must come in complete Cartesian shells (basis_construction order)";
+            return -1;
+        }
+        ShellH sh;
+        sh.c[0] = xyz[3 * mu];
+        sh.c[1] = xyz[3 * mu + 1];
+        sh.c[2] = xyz[3 * mu + 2];
+        sh.L = L;
+        sh.nprim = np;
+        sh.poff = (int)bp.alpha.size();
+        sh.fn0 = mu;
+        for (int k = 0; k < n; ++k) {
+            const int f = mu + k;
+            const bool okc = lmn[3 * f] == cart_x(L, k) && lmn[3 * f + 1] == cart_y(L, k) && lmn[3 * f + 2] == cart_z(L, k);
+            const bool okp = poff[f + 1] - poff[f] == np && xyz[3 * f] == sh.c[0] && xyz[3 * f + 1] == sh.c[1] &&
+                             xyz[3 * f + 2] == sh.c[2];
+            if (!okc || !okp) {
+                err = "basis functions must come in complete Cartesian shells in angmom_partition order";
+                return -1;
+            }
+            for (int i = 0; i < np; ++i) {
+                if (alpha[poff[f] + i] != alpha[poff[mu] + i] || coeff[poff[f] + i] != coeff[poff[mu] + i]) {
+                    err = "components of one shell must share exponents and coefficients";
+                    return -1;
+                }
+            }
+            // normalisation must factor as radial(alpha_i) * component scale
+            const double s = norm[poff[f]] / norm[poff[mu]];
+            for (int i = 0; i < np; ++i) {
+                const double r = norm[poff[mu] + i] * s;
+                if (std::fabs(r - norm[poff[f] + i]) > 1e-13 * std::fabs(norm[poff[f] + i])) {
+                    err = "primitive normalisation does not factor into radial and component parts";
+                    return -1;
+                }
+            }
+            bp.fscale[f] = s;
+        }
+        for (int i = 0; i < np; ++i) {
+            if (!(alpha[poff[mu] + i] > 0.0)) {
+                err = "primitive exponents must be positive";
+                return -1;
+            }
+            bp.alpha.push_back(alpha[poff[mu] + i]);
+            bp.w.push_back(coeff[poff[mu] + i] * norm[poff[mu] + i]);
+        }
+        bp.shells.push_back(sh);
+        mu += n;
+    }
+    bp.nshell = (int)bp.shells.size();
+    bp.npair = bp.nshell * (bp.nshell + 1) / 2;
+    bp.hdr.resize(bp.npair);
+    bp.pclass.resize(bp.npair);
+    for (int A = 0; A < bp.nshell; ++A)
+        for (int B = 0; B <= A; ++B) {
+            const int x = A * (A + 1) / 2 + B;
+            int s1 = A, s2 = B;
+            if (bp.shells[A].L < bp.shells[B].L) std::swap(s1, s2);
+            const ShellH& S1 = bp.shells[s1];
+            const ShellH& S2 = bp.shells[s2];
+            PairHdr h;
+            h.s1 = s1;
+            h.s2 = s2;
+            h.fn1 = S1.fn0;
+            h.fn2 = S2.fn0;
+            h.diag = (A == B);
+            h.canon = x;
+            h.prim_off = (int)bp.ppd.size();
+            const double AB2 = (S1.c[0] - S2.c[0]) * (S1.c[0] - S2.c[0]) + (S1.c[1] - S2.c[1]) * (S1.c[1] - S2.c[1]) +
+                               (S1.c[2] - S2.c[2]) * (S1.c[2] - S2.c[2]);
+            int kept = 0;
+            for (int i = 0; i < S1.nprim; ++i)
+                for (int j = 0; j < S2.nprim; ++j) {
+                    const double a = bp.alpha[S1.poff + i], b = bp.alpha[S2.poff + j];
+                    const double p = a + b, m = a * b / p;
+                    if (m * AB2 > PRIM_PAIR_CUT) continue;
+                    PrimPair<double> d;
+                    d.p = p;
+                    d.i2p = 0.5 / p;
+                    d.w = bp.w[S1.poff + i] * bp.w[S2.poff + j] * std::exp(-m * AB2) / p;
+                    for (int x3 = 0; x3 < 3; ++x3) {
+                        d.P[x3] = (a * S1.c[x3] + b * S2.c[x3]) / p;
+                        d.PA[x3] = d.P[x3] - S1.c[x3];
+                        d.PB[x3] = d.P[x3] - S2.c[x3];
+                    }
+                    bp.ppd.push_back(d);
+                    PrimPair<float> f;
+                    f.p = (float)d.p;
+                    f.i2p = (float)d.i2p;
+                    f.w = (float)d.w;
+                    for (int x3 = 0; x3 < 3; ++x3) {
+                        f.P[x3] = (float)d.P[x3];
+                        f.PA[x3] = (float)d.PA[x3];
+                        f.PB[x3] = (float)d.PB[x3];
+                    }
+                    bp.ppf.push_back(f);
+                    ++kept;
+                }
+            h.nprim = kept;
+            bp.hdr[x] = h;
+            bp.pclass[x] = pair_class(S1.L, S2.L);
+        }
+    const int P = K * (K + 1) / 2;
+    bp.fnpairs.resize(P);
+    for (int i = 0; i < K; ++i)
+        for (int j = 0; j <= i; ++j) bp.fnpairs[i * (i + 1) / 2 + j] = make_int2(i, j);
+    return 0;
+}
+
+// Shell-pair maximum of the function-pair Schwarz bounds.
+inline void shell_pair_bounds(const BasisPlan& bp, const double* fnb, std::vector<double>& Q) {
+    Q.assign(bp.npair, 0.0);
+    for (int x = 0; x < bp.npair; ++x) {
+        const PairHdr& h = bp.hdr[x];
+        const int n1 = ncart(bp.shells[h.s1].L), n2 = ncart(bp.shells[h.s2].L);
+        double m = 0.0;
+        for (int a = 0; a < n1; ++a)
+            for (int b = 0; b < n2; ++b) m = std::max(m, fnb[pair_of(h.fn1 + a, h.fn2 + b)]);
+        Q[x] = m;
+    }
+}
+
+struct QuartetLists {
+    std::vector<uint2> list[NQCLASS][2];   // [class][0 = FP64, 1 = FP32]
+    size_t total() const {
+        size_t n = 0;
+        for (int c = 0; c < NQCLASS; ++c) n += list[c][0].size() + list[c][1].size();
+        return n;
+    }
+};
+
+// Screened canonical shell quartets (x >= y over canonical pair ids), split by
+// class and precision.  A quartet is kept when Q_x Q_y >= threshold (no
+// function quartet inside it can pass the function-level mask otherwise).
+// In mixed precision, quartets with Q_x Q_y < split go to the FP32 list.
+// Within a class, quartets are ordered by primitive work so that lanes of a
+// wavefront run loops of similar length.
+inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, double thr, bool mixed, double split,
+                          QuartetLists& ql) {
+    for (int c = 0; c < NQCLASS; ++c) {
+        ql.list[c][0].clear();
+        ql.list[c][1].clear();
+    }
+    for (int x = 0; x < bp.npair; ++x) {
+        if (bp.hdr[x].nprim == 0) continue;
+        for (int y = 0; y <= x; ++y) {
+            if (bp.hdr[y].nprim == 0) continue;
+            const double qq = Q[x] * Q[y];
+            if (qq < thr) continue;
+            int bra = x, ket = y;
+            if (bp.pclass[x] < bp.pclass[y]) std::swap(bra, ket);
+            const int qc = quartet_class(bp.pclass[bra], bp.pclass[ket]);
+            const int prec = (mixed && qq < split) ? 1 : 0;
+            ql.list[qc][prec].push_back(make_uint2((unsigned)bra, (unsigned)ket));
+        }
+    }
+    for (int c = 0; c < NQCLASS; ++c)
+        for (int p = 0; p < 2; ++p) {
+            auto& v = ql.list[c][p];
+            std::stable_sort(v.begin(), v.end(), [&](const uint2& a, const uint2& b) {
+                const int wa = bp.hdr[a.x].nprim * bp.hdr[a.y].nprim, wb = bp.hdr[b.x].nprim * bp.hdr[b.y].nprim;
+                return wa > wb;
+            });
+        }
+}
+
+// Diagonal quartets (x|x) for the Schwarz bounds.
+inline void plan_diag(const BasisPlan& bp, std::vector<uint2> (&lists)[NQCLASS]) {
+    for (int c = 0; c < NQCLASS; ++c) lists[c].clear();
+    for (int x = 0; x < bp.npair; ++x) {
+        if (bp.hdr[x].nprim == 0) continue;
+        lists[quartet_class(bp.pclass[x], bp.pclass[x])].push_back(make_uint2((unsigned)x, (unsigned)x));
+    }
+}
+
+// F_n(T_i) on the grid T_i = i h, n = 0..BOYS_NCOL-1, from the positive series
+// F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) in long double.
+inline void boys_table(std::vector<double>& tab) {
+    tab.assign((size_t)BOYS_NGRID * BOYS_NCOL, 0.0);
+    for (int i = 0; i < BOYS_NGRID; ++i) {
+        const long double T = (long double)i * (long double)BOYS_H;
+        const long double e = expl(-T);
+        for (int n = 0; n < BOYS_NCOL; ++n) {
+            long double term = 1.0L / (2.0L * n + 1.0L), sum = term;
+            for (int k = 1; k < 4000; ++k) {
+                term *= 2.0L * T / (2.0L * n + 2.0L * k + 1.0L);
+                sum += term;
+                if (term < 1e-24L * sum) break;
+            }
+            tab[(size_t)i * BOYS_NCOL + n] = (double)(e * sum);
+        }
+    }
+}
+
+inline HermTab herm_tab() {
+    HermTab h{};
+    int w = 0;
+    for (int s = 0; s <= 12; ++s)
+        for (int t = 0; t <= s; ++t)
+            for (int u = 0; u <= s - t; ++u) {
+                h.t[w] = (unsigned char)t;
+                h.u[w] = (unsigned char)u;
+                h.v[w] = (unsigned char)(s - t - u);
+                ++w;
+            }
+    return h;
+}
+
+}  // namespace hfx

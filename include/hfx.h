@@ -1,0 +1,128 @@
+/*
+ * hfx.h -- C ABI of the MI355X (gfx950) ERI + Fock engine.
+ *
+ * Drop-in boundary for the GPU path of KLYYCYNN/hartree_fock_cpp.  Plain C
+ * types only: host pointers unless a name ends in _device, sizes as size_t,
+ * status codes as int (0 = success, negative = error class; the message is
+ * available from hfx_last_error()).  All functions are synchronous unless a
+ * stream argument is taken.  Thread compatibility: distinct hfx_basis /
+ * hfx_fock objects may be used from distinct threads concurrently.
+ *
+ * Conventions reproduced from the reference (all cited file:line are in
+ * KLYYCYNN/hartree_fock_cpp):
+ *   - basis functions are Cartesian, in basis_construction order
+ *     (include/basis.hpp:157-262): whole shells, components in
+ *     angmom_partition order (basis.hpp:52-64), L <= 3 (cuda/include/recursion.cuh:5);
+ *   - primitives carry (alpha, coeff, norm) with norm = cart_norm per
+ *     component (basis.hpp:28-49); the library never renormalises;
+ *   - the unique ERI array has Q = P(P+1)/2 entries, P = K(K+1)/2, value of
+ *     (ij|kl) at eri_index(i,j,k,l) (include/indexing.hpp:33-52); entries whose
+ *     Schwarz product b_IJ*b_KL < threshold are exactly 0.0
+ *     (include/integrals.hpp:771-776, cuda/include/kernels.cuh:104-110);
+ *   - matrices are K x K row-major doubles (include/indexing.hpp:4-7).
+ */
+#ifndef HFX_H
+#define HFX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFX_OK 0
+#define HFX_ERR_INVALID (-1)      /* bad argument / unsupported basis layout */
+#define HFX_ERR_DEVICE (-2)       /* HIP runtime error */
+#define HFX_ERR_NOMEM (-3)        /* device allocation failed or over budget */
+#define HFX_ERR_UNSUPPORTED (-4)  /* angular momentum or size beyond limits */
+
+#define HFX_MODE_STORED 0   /* unique ERIs resident in HBM, J/K streamed from them */
+#define HFX_MODE_DIRECT 1   /* ERIs recomputed and contracted every build */
+#define HFX_MODE_AUTO 2     /* stored if it fits the HBM budget, else direct */
+
+#define HFX_PREC_FP64 0     /* oracle-matched drop-in form */
+#define HFX_PREC_MIXED 1    /* FP32 for quartets with Schwarz bound < split, FP64 otherwise */
+
+typedef struct hfx_basis hfx_basis;
+typedef struct hfx_fock hfx_fock;
+
+/* Last error message of the calling thread ("" when none). */
+const char* hfx_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int hfx_device_count(void);
+
+/* Replaces convert_basis_to_gpu (include/gpu_types.hpp:21-52) and the
+ * per-call H2D copy of ERI_GPU (cuda/src/eri.cu:40-62): builds the device
+ * resident shell, shell-pair and primitive-pair tables once.
+ * Function mu has centre xyz[3mu..3mu+2], exponents lmn[3mu..3mu+2] and
+ * primitives poff[mu] .. poff[mu+1]-1 of alpha/coeff/norm (the fields of
+ * primitive_gaussian, include/types.hpp:16-20). */
+int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, const double* alpha,
+                     const double* coeff, const double* norm, hfx_basis** out);
+void hfx_basis_destroy(hfx_basis* b);
+
+/* Shell / shell-pair / kept primitive-pair counts of a basis. */
+int hfx_basis_info(const hfx_basis* b, int* nshell, int* npair, int* nprimpair);
+
+/* Replaces schwarz_bounds (include/integrals.hpp:732-752), which the
+ * reference runs on the host inside ERI_GPU (cuda/src/eri.cu:32):
+ * bounds[pair_index(i,j)] = sqrt(max((ij|ij), 0)), K(K+1)/2 entries. */
+int hfx_schwarz_bounds(hfx_basis* b, double* bounds);
+
+/* Replaces ERI_GPU (cuda/src/eri.cu:20-89, declared include/scf.hpp:41-44):
+ * the full unique ERI array (Q entries, host memory) in the eri_index layout,
+ * FP64, function-level Schwarz mask. */
+int hfx_eri_unique(hfx_basis* b, double threshold, double* out, size_t Q);
+
+/* Same, into caller-owned device memory (Q doubles) on `stream`
+ * (a hipStream_t, NULL = default stream); no host round trip. */
+int hfx_eri_unique_device(hfx_basis* b, double threshold, double* d_out, size_t Q, void* stream);
+
+/* Replaces repulsion_matrix (include/linalg.hpp:100-129) for a
+ * device-resident unique array: G = J - 1/2 K with
+ * G_ij = sum_kl P_kl [(ij|kl) - 1/2 (ik|jl)]. P, G: host K x K. */
+int hfx_repulsion_matrix(hfx_basis* b, const double* d_eri, const double* P, double* G);
+
+/* Replaces coulomb_matrix (include/linalg.hpp:215-257) and exchange_matrix
+ * (include/linalg.hpp:260-296) for a device-resident unique array, in one
+ * pass: J from Pa + Pb, Ka from Pa, Kb from Pb (host K x K). */
+int hfx_coulomb_exchange(hfx_basis* b, const double* d_eri, const double* Pa, const double* Pb, double* J,
+                         double* Ka, double* Kb);
+
+typedef struct {
+    int mode;              /* HFX_MODE_* */
+    int precision;         /* HFX_PREC_* */
+    double threshold;      /* function-level Schwarz threshold (reference screening_threshold) */
+    double mixed_split;    /* quartet bound below which the mixed form uses FP32 (default 1e-5) */
+    size_t hbm_budget;     /* bytes this job may hold on the device; 0 = unlimited */
+} hfx_fock_opts;
+
+/* Fock-build context for one SCF job: owns its stored ERIs (stored mode) or
+ * quartet lists (direct mode), device densities and accumulators. */
+int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out);
+void hfx_fock_destroy(hfx_fock* f);
+
+/* Selected mode (HFX_MODE_STORED or HFX_MODE_DIRECT) and device bytes held. */
+int hfx_fock_info(const hfx_fock* f, int* mode, size_t* device_bytes, size_t* n_quartets);
+
+/* One Fock build.  RHF: Pb == NULL, J from Pa and Ka from Pa (the caller forms
+ * G = J - 1/2 Ka, as repulsion_matrix does).  UHF: J from Pa + Pb, Ka from Pa,
+ * Kb from Pb (coulomb_matrix / exchange_matrix).  All host K x K; J, Ka, Kb
+ * may be NULL when not wanted. */
+int hfx_fock_build(hfx_fock* f, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb);
+
+/* Same on caller-owned device matrices (K x K), on `stream`. */
+int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka,
+                          double* d_Kb, void* stream);
+
+/* Per-class launch statistics of the last build: for each of the 55 classes
+ * (ss|ss)..(ff|ff) (index qc = pb(pb+1)/2 + pk over pair classes
+ * ss,ps,pp,ds,dp,dd,fs,fp,fd,ff), the number of FP64 and FP32 shell quartets. */
+int hfx_fock_class_counts(const hfx_fock* f, long long* n_fp64, long long* n_fp32);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFX_H */

@@ -1,0 +1,209 @@
+// cpu_emu.cpp -- test-only CPU build of the product's ERI kernel bodies.
+//
+// Compiles hartree_fock_cpp_amd/csrc/device/{plan,eri_body}.hpp for the host
+// and runs every shell quartet with one "lane" (nlanes = 1), so the planner,
+// class dispatch, Hermite/R recursions, contractions and all three epilogues
+// can be checked against the oracle without a GPU.  It is never loaded by the
+// product library, smoke() or bench.py.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../hartree_fock_cpp_amd/csrc/device/plan.hpp"
+
+using namespace hfx;
+
+namespace {
+
+std::string g_err;
+
+struct EmuCtx {
+    BasisPlan plan;
+    std::vector<double> boysd;
+    std::vector<float> boysf;
+    HermTab ht;
+    std::vector<double> bounds, Qsh;
+};
+
+template <int QC, class Real>
+void run_class(const EmuCtx& c, const std::vector<uint2>& list, const EpiArgs& ep) {
+    using C = QClass<QC>;
+    const PrimPair<Real>* pp;
+    const Real* tab;
+    if constexpr (sizeof(Real) == 8) {
+        pp = c.plan.ppd.data();
+        tab = c.boysd.data();
+    } else {
+        pp = c.plan.ppf.data();
+        tab = c.boysf.data();
+    }
+    if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
+        for (const uint2& e : list) family_a_one<C::LA, C::LB, C::LC, C::LD, Real>(e, c.plan.hdr.data(), pp, tab, ep);
+    } else {
+        auto sm = std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
+        auto sync = []() {};
+        for (const uint2& e : list)
+            family_b_one<C::LA, C::LB, C::LC, C::LD, Real>(e, c.plan.hdr.data(), pp, tab, ep, *sm, c.ht, 0, 1, sync);
+    }
+}
+
+using RunFn = void (*)(const EmuCtx&, const std::vector<uint2>&, const EpiArgs&);
+
+template <class Real, int... QC>
+RunFn pick(std::integer_sequence<int, QC...>, int qc) {
+    static const RunFn t[] = {&run_class<QC, Real>...};
+    return t[qc];
+}
+
+void run_lists(const EmuCtx& c, const QuartetLists& ql, const EpiArgs& ep) {
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        if (!ql.list[qc][0].empty()) pick<double>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][0], ep);
+        if (!ql.list[qc][1].empty()) pick<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][1], ep);
+    }
+}
+
+EpiArgs base_epi(const EmuCtx& c) {
+    EpiArgs ep{};
+    ep.K = c.plan.K;
+    ep.bounds = c.bounds.data();
+    ep.fscale = c.plan.fscale.data();
+    ep.nspin = 1;
+    return ep;
+}
+
+int make_ctx(int K, const double* xyz, const int* lmn, const int* poff, const double* alpha, const double* coeff,
+             const double* norm, EmuCtx& c) {
+    if (build_plan(K, xyz, lmn, poff, alpha, coeff, norm, c.plan, g_err)) return -1;
+    boys_table(c.boysd);
+    c.boysf.assign(c.boysd.begin(), c.boysd.end());
+    c.ht = herm_tab();
+    const int P = K * (K + 1) / 2;
+    c.bounds.assign(P, 0.0);
+    std::vector<uint2> diag[NQCLASS];
+    plan_diag(c.plan, diag);
+    QuartetLists ql;
+    for (int q = 0; q < NQCLASS; ++q) ql.list[q][0] = diag[q];
+    std::vector<double> out(P, 0.0);
+    EpiArgs ep = base_epi(c);
+    ep.mode = EPI_DIAG;
+    ep.out = out.data();
+    run_lists(c, ql, ep);
+    c.bounds = out;
+    shell_pair_bounds(c.plan, c.bounds.data(), c.Qsh);
+    return 0;
+}
+
+}  // namespace
+
+#define EMU_BASIS_ARGS int K, const double *xyz, const int *lmn, const int *poff, const double *alpha, \
+                       const double *coeff, const double *norm
+
+extern "C" {
+
+const char* emu_last_error() { return g_err.c_str(); }
+
+int emu_schwarz(EMU_BASIS_ARGS, double* bounds) {
+    EmuCtx c;
+    if (make_ctx(K, xyz, lmn, poff, alpha, coeff, norm, c)) return -1;
+    std::memcpy(bounds, c.bounds.data(), c.bounds.size() * sizeof(double));
+    return 0;
+}
+
+// Full unique array (Q doubles, zero-filled by the caller).
+int emu_eri_unique(EMU_BASIS_ARGS, double threshold, double* out) {
+    EmuCtx c;
+    if (make_ctx(K, xyz, lmn, poff, alpha, coeff, norm, c)) return -1;
+    QuartetLists ql;
+    plan_quartets(c.plan, c.Qsh, threshold, false, 0.0, ql);
+    EpiArgs ep = base_epi(c);
+    ep.mode = EPI_UNIQUE;
+    ep.threshold = threshold;
+    ep.out = out;
+    run_lists(c, ql, ep);
+    return 0;
+}
+
+// Direct J/K (symmetrised).  Pb == NULL: RHF (J and Ka from Pa).
+int emu_fock_direct(EMU_BASIS_ARGS, double threshold, int mixed, double split, const double* Pa, const double* Pb,
+                    double* J, double* Ka, double* Kb) {
+    EmuCtx c;
+    if (make_ctx(K, xyz, lmn, poff, alpha, coeff, norm, c)) return -1;
+    QuartetLists ql;
+    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, ql);
+    const size_t K2 = (size_t)K * K;
+    std::vector<double> Dj(Pa, Pa + K2);
+    if (Pb)
+        for (size_t i = 0; i < K2; ++i) Dj[i] += Pb[i];
+    std::memset(J, 0, K2 * sizeof(double));
+    std::memset(Ka, 0, K2 * sizeof(double));
+    if (Pb) std::memset(Kb, 0, K2 * sizeof(double));
+    EpiArgs ep = base_epi(c);
+    ep.mode = EPI_JK;
+    ep.threshold = threshold;
+    ep.nspin = Pb ? 2 : 1;
+    ep.Dj = Dj.data();
+    ep.Da = Pa;
+    ep.Db = Pb;
+    ep.J = J;
+    ep.Ka = Ka;
+    ep.Kb = Kb;
+    run_lists(c, ql, ep);
+    auto sym = [&](double* M) {
+        for (int i = 0; i < K; ++i)
+            for (int j = 0; j <= i; ++j) {
+                const double s = M[i * K + j] + M[j * K + i];
+                M[i * K + j] = s;
+                M[j * K + i] = s;
+            }
+    };
+    sym(J);
+    sym(Ka);
+    if (Pb) sym(Kb);
+    return 0;
+}
+
+// Number of shell quartets per class (fp64, fp32) for a threshold/split.
+int emu_class_counts(EMU_BASIS_ARGS, double threshold, int mixed, double split, long long* n64, long long* n32) {
+    EmuCtx c;
+    if (make_ctx(K, xyz, lmn, poff, alpha, coeff, norm, c)) return -1;
+    QuartetLists ql;
+    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, ql);
+    for (int q = 0; q < NQCLASS; ++q) {
+        n64[q] = (long long)ql.list[q][0].size();
+        n32[q] = (long long)ql.list[q][1].size();
+    }
+    return 0;
+}
+
+// Boys values from the product's table path (FP64 and FP32).
+void emu_boys(int mmax, double T, double* F64, float* F32) {
+    static std::vector<double> tab;
+    static std::vector<float> tabf;
+    if (tab.empty()) {
+        boys_table(tab);
+        tabf.assign(tab.begin(), tab.end());
+    }
+    double Fd[BOYS_MMAX + 1];
+    float Ff[BOYS_MMAX + 1];
+    // dispatch on mmax at run time through a small switch of template instances
+    switch (mmax) {
+#define EMU_B(M)                                          \
+    case M:                                               \
+        boys_fm<M, double>(T, tab.data(), Fd);            \
+        boys_fm<M, float>((float)T, tabf.data(), Ff);     \
+        break;
+        EMU_B(0) EMU_B(1) EMU_B(2) EMU_B(3) EMU_B(4) EMU_B(5) EMU_B(6) EMU_B(7) EMU_B(8) EMU_B(9) EMU_B(10) EMU_B(11)
+        EMU_B(12)
+#undef EMU_B
+        default:
+            return;
+    }
+    for (int m = 0; m <= mmax; ++m) {
+        F64[m] = Fd[m];
+        F32[m] = Ff[m];
+    }
+}
+
+}  // extern "C"
