@@ -1,0 +1,66 @@
+// eri_launch_impl.hpp -- __global__ ERI kernels and per-class launchers.
+//
+// Included by the eri_part*.hip translation units; each instantiates a range
+// of the 55 classes so the build parallelises.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "eri_body.hpp"
+#include "eri_classes.hpp"
+#include "hfx_launch.hpp"
+
+namespace hfx {
+
+template <int LA, int LB, int LC, int LD, class Real>
+__global__ __launch_bounds__(128) void k_fam_a(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
+                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
+                                                EpiArgs ep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    family_a_one<LA, LB, LC, LD, Real>(list[i], hdr, pp, tab, ep);
+}
+
+template <int LA, int LB, int LC, int LD, class Real>
+__global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
+                                               const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
+                                               const HermTab* __restrict__ ht, EpiArgs ep) {
+    __shared__ FamBSmem<LA, LB, LC, LD, Real> sm;
+    auto sync = []() { __syncthreads(); };
+    for (int q = blockIdx.x; q < n; q += gridDim.x)
+        family_b_one<LA, LB, LC, LD, Real>(list[q], hdr, pp, tab, ep, sm, *ht, (int)threadIdx.x, 64, sync);
+}
+
+template <int QC, class Real>
+hipError_t launch_class(const ClassLaunch& L, hipStream_t st) {
+    using C = QClass<QC>;
+    if (L.n <= 0) return hipSuccess;
+    const PrimPair<Real>* pp = static_cast<const PrimPair<Real>*>(L.prim);
+    const Real* tab = static_cast<const Real*>(L.boys);
+    if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
+        const int block = 128;
+        const int grid = (L.n + block - 1) / block;
+        k_fam_a<C::LA, C::LB, C::LC, C::LD, Real><<<grid, block, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep);
+    } else {
+        const int grid = L.n < 65536 ? L.n : 65536;
+        k_fam_b<C::LA, C::LB, C::LC, C::LD, Real><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.herm, L.ep);
+    }
+    return hipGetLastError();
+}
+
+template <int Begin, int... I>
+hipError_t launch_range(std::integer_sequence<int, I...>, int qc, bool fp32, const ClassLaunch& L, hipStream_t st) {
+    static constexpr LaunchFn t64[] = {&launch_class<Begin + I, double>...};
+    static constexpr LaunchFn t32[] = {&launch_class<Begin + I, float>...};
+    return fp32 ? t32[qc - Begin](L, st) : t64[qc - Begin](L, st);
+}
+
+}  // namespace hfx
+
+#define HFX_DEFINE_PART(NAME, BEGIN, END)                                                               \
+    namespace hfx {                                                                                     \
+    hipError_t NAME(int qc, bool fp32, const ClassLaunch& L, hipStream_t st) {                         \
+        return launch_range<BEGIN>(std::make_integer_sequence<int, (END) - (BEGIN)>{}, qc, fp32, L, st); \
+    }                                                                                                   \
+    }

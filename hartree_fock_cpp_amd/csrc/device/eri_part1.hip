@@ -1,0 +1,3 @@
+// eri_part1.hip -- instantiates the ERI kernels of classes [15, 28).
+#include "eri_launch_impl.hpp"
+HFX_DEFINE_PART(launch_eri_part1, 15, 28)

@@ -42,6 +42,7 @@ struct DeviceConsts {
     std::string err;
     double* boys_d = nullptr;
     float* boys_f = nullptr;
+    HermTab* herm = nullptr;
 };
 DeviceConsts g_consts;
 
@@ -54,7 +55,9 @@ int ensure_consts() {
         if (e == hipSuccess) e = hipMalloc(&g_consts.boys_f, tabf.size() * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(g_consts.boys_d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(g_consts.boys_f, tabf.data(), tabf.size() * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = upload_herm_tab(herm_tab());
+        if (e == hipSuccess) e = hipMalloc(&g_consts.herm, sizeof(HermTab));
+        const HermTab ht = herm_tab();
+        if (e == hipSuccess) e = hipMemcpy(g_consts.herm, &ht, sizeof(HermTab), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             g_consts.status = HFX_ERR_DEVICE;
             g_consts.err = std::string("device constant setup: ") + hipGetErrorString(e);
@@ -138,6 +141,7 @@ int launch_lists(const hfx_basis* b, const uint2* d_lists, const size_t (*off)[2
             L.hdr = b->d_hdr;
             L.prim = p ? (const void*)b->d_ppf : (const void*)b->d_ppd;
             L.boys = p ? (const void*)g_consts.boys_f : (const void*)g_consts.boys_d;
+            L.herm = g_consts.herm;
             L.ep = ep;
             HIPCHK(launch_eri_class(qc, p == 1, L, st));
         }

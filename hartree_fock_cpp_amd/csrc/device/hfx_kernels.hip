@@ -1,15 +1,13 @@
 // hfx_kernels.hip -- __global__ entry points for gfx950 and their launch table.
 //
-//  * k_fam_a<class, Real>   : one lane per shell quartet (low-L classes)
-//  * k_fam_b<class, Real>   : one 64-lane wavefront per shell quartet (the rest)
+//  * class dispatch to the ERI kernels instantiated in eri_part0..4.hip
+//    (k_fam_a: one lane per shell quartet; k_fam_b: one wavefront per shell quartet)
 //  * k_jk_unique            : J/K from the stored 8-fold unique ERI array
-//  * k_fock_finish          : J = Jacc + Jacc^T, K = Kacc + Kacc^T, G / F assembly
+//  * k_symmetrize, k_zero, k_axpy : small helpers of the Fock build
 //
 // Every ERI kernel reads a class-homogeneous list of (bra pair, ket pair)
 // ids, so one launch covers one angular class and one precision.
 #include <hip/hip_runtime.h>
-
-#include <utility>
 
 #include "eri_body.hpp"
 #include "eri_classes.hpp"
@@ -17,63 +15,26 @@
 
 namespace hfx {
 
-__constant__ HermTab c_herm_tab;
+hipError_t launch_eri_part0(int, bool, const ClassLaunch&, hipStream_t);
+hipError_t launch_eri_part1(int, bool, const ClassLaunch&, hipStream_t);
+hipError_t launch_eri_part2(int, bool, const ClassLaunch&, hipStream_t);
+hipError_t launch_eri_part3(int, bool, const ClassLaunch&, hipStream_t);
+hipError_t launch_eri_part4(int, bool, const ClassLaunch&, hipStream_t);
 
-template <int LA, int LB, int LC, int LD, class Real>
-__global__ __launch_bounds__(128) void k_fam_a(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
-                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
-                                                EpiArgs ep) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    family_a_one<LA, LB, LC, LD, Real>(list[i], hdr, pp, tab, ep);
-}
-
-template <int LA, int LB, int LC, int LD, class Real>
-__global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
-                                               const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
-                                               EpiArgs ep) {
-    __shared__ FamBSmem<LA, LB, LC, LD, Real> sm;
-    auto sync = []() { __syncthreads(); };
-    for (int q = blockIdx.x; q < n; q += gridDim.x)
-        family_b_one<LA, LB, LC, LD, Real>(list[q], hdr, pp, tab, ep, sm, c_herm_tab, (int)threadIdx.x, 64, sync);
-}
-
-template <int QC, class Real>
-static hipError_t launch_class(const ClassLaunch& L, hipStream_t st) {
-    using C = QClass<QC>;
-    if (L.n <= 0) return hipSuccess;
-    const PrimPair<Real>* pp = static_cast<const PrimPair<Real>*>(L.prim);
-    const Real* tab = static_cast<const Real*>(L.boys);
-    if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
-        const int block = 128;
-        const int grid = (L.n + block - 1) / block;
-        k_fam_a<C::LA, C::LB, C::LC, C::LD, Real><<<grid, block, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep);
-    } else {
-        const int grid = L.n < 65536 ? L.n : 65536;
-        k_fam_b<C::LA, C::LB, C::LC, C::LD, Real><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep);
-    }
-    return hipGetLastError();
-}
-
-template <class Real, int... QC>
-static constexpr LaunchFn make_table(std::integer_sequence<int, QC...>, int qc) {
-    constexpr LaunchFn t[] = {&launch_class<QC, Real>...};
-    return t[qc];
-}
-
+// Class ranges instantiated by eri_part0..4.hip.
 hipError_t launch_eri_class(int qc, bool fp32, const ClassLaunch& L, hipStream_t st) {
     if (qc < 0 || qc >= NQCLASS) return hipErrorInvalidValue;
-    LaunchFn f = fp32 ? make_table<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)
-                      : make_table<double>(std::make_integer_sequence<int, NQCLASS>{}, qc);
-    return f(L, st);
+    if (qc < 15) return launch_eri_part0(qc, fp32, L, st);
+    if (qc < 28) return launch_eri_part1(qc, fp32, L, st);
+    if (qc < 38) return launch_eri_part2(qc, fp32, L, st);
+    if (qc < 47) return launch_eri_part3(qc, fp32, L, st);
+    return launch_eri_part4(qc, fp32, L, st);
 }
 
 bool class_uses_family_a(int qc) {
     int pb = qc_bra(qc), pk = qc_ket(qc);
     return use_family_a(pc_la(pb), pc_lb(pb), pc_la(pk), pc_lb(pk));
 }
-
-hipError_t upload_herm_tab(const HermTab& h) { return hipMemcpyToSymbol(HIP_SYMBOL(c_herm_tab), &h, sizeof(HermTab)); }
 
 // ------------------------------------------------------------------ stored-ERI J/K
 // One workgroup per function-pair row IJ = pair(i, j) of the unique array

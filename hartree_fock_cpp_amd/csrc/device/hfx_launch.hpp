@@ -12,6 +12,7 @@ struct ClassLaunch {
     const PairHdr* hdr;    // device pair headers (indexed by canonical pair id)
     const void* prim;      // device PrimPair<Real> array
     const void* boys;      // device Boys table (Real)
+    const HermTab* herm;   // device Hermite (t,u,v) decode table
     EpiArgs ep;
 };
 
@@ -19,7 +20,6 @@ using LaunchFn = hipError_t (*)(const ClassLaunch&, hipStream_t);
 
 hipError_t launch_eri_class(int qc, bool fp32, const ClassLaunch& L, hipStream_t st);
 bool class_uses_family_a(int qc);
-hipError_t upload_herm_tab(const HermTab& h);
 hipError_t launch_jk_unique(const double* eri, const int2* pairs, int K, int nspin, const double* Dj, const double* Da,
                             const double* Db, double* J, double* Ka, double* Kb, hipStream_t st);
 hipError_t launch_symmetrize(double* M, int K, hipStream_t st);
