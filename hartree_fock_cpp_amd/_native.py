@@ -29,6 +29,12 @@ def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"native library {LIB_PATH} is missing; build it with "
                           "`python -c 'import __graft_entry__ as g; g.build()'` from the repository root")
+    # torch ships its own libamdhip64.so.7; importing it first makes the engine
+    # bind to that same HIP runtime instead of starting a second one in-process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return C.CDLL(LIB_PATH)
 
 

@@ -147,16 +147,28 @@ struct FamA {
         });
     }
 
-    // Full primitive loop for one shell quartet; acc must be zeroed.
+    // Primitive loop of one shell quartet over the flattened (bra, ket)
+    // primitive-pair index f = i * nk + j, taking f = sub, sub + G, ... so
+    // that a group of G lanes shares the quartet; acc must be zeroed and holds
+    // this lane's partial sum.
     static HFX_HD __forceinline__ void quartet(const PairHdr& hb, const PairHdr& hk, const PrimPair<Real>* pp,
-                                               const Real* tab, Real* acc) {
-        for (int i = 0; i < hb.nprim; ++i) {
-            const PrimPair<Real> b = pp[hb.prim_off + i];
-            Herm1D<LA, LB, Real> ex, ey, ez;
-            ex.build(b.i2p, b.PA[0], b.PB[0]);
-            ey.build(b.i2p, b.PA[1], b.PB[1]);
-            ez.build(b.i2p, b.PA[2], b.PB[2]);
-            for (int j = 0; j < hk.nprim; ++j) {
+                                               const Real* tab, Real* acc, int sub, int G) {
+        const int nk = hk.nprim;
+        const int tot = hb.nprim * nk;
+        int i = nk > 0 ? sub / nk : 0;
+        int j = sub - i * nk;
+        int last_i = -1;
+        PrimPair<Real> b;
+        Herm1D<LA, LB, Real> ex, ey, ez;
+        for (int f = sub; f < tot; f += G) {
+            if (i != last_i) {
+                b = pp[hb.prim_off + i];
+                ex.build(b.i2p, b.PA[0], b.PB[0]);
+                ey.build(b.i2p, b.PA[1], b.PB[1]);
+                ez.build(b.i2p, b.PA[2], b.PB[2]);
+                last_i = i;
+            }
+            {
                 const PrimPair<Real> k = pp[hk.prim_off + j];
                 Herm1D<LC, LD, Real> fx, fy, fz;
                 fx.build(k.i2p, k.PA[0], k.PB[0]);
@@ -185,15 +197,20 @@ struct FamA {
                 Real pref = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
                 Real a[L + 1];
                 Real m2r = Real(-2) * rho;
-                Real f = pref;
+                Real fct = pref;
 #pragma unroll
                 for (int n = 0; n <= L; ++n) {
-                    a[n] = f * F[n];
-                    f *= m2r;
+                    a[n] = fct * F[n];
+                    fct *= m2r;
                 }
                 Real R[nherm(L)];
                 hermite_R<L, Real>(a, X, Y, Z, R);
                 contract(acc, ex, ey, ez, fx, fy, fz, R);
+            }
+            j += G;
+            while (j >= nk) {
+                j -= nk;
+                ++i;
             }
         }
     }
@@ -321,17 +338,54 @@ HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk)
     return q;
 }
 
-template <int LA, int LB, int LC, int LD, class Real>
-HFX_HD __forceinline__ void family_a_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
-                                         const EpiArgs& ep) {
+// One shell quartet per group of G lanes (G = 1, 4, 16 or 64, a power of two
+// dividing the wavefront).  `sub` is the lane's index inside its group; every
+// lane of the group must call this (inactive groups pass active = false) so
+// the cross-lane reduction sees the whole group.  Lane 0 of the group runs the
+// epilogue.
+template <int LA, int LB, int LC, int LD, class Real, int G>
+HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const PairHdr* hdr, const PrimPair<Real>* pp,
+                                           const Real* tab, const EpiArgs& ep) {
     using F = FamA<LA, LB, LC, LD, Real>;
     const PairHdr hb = hdr[e.x], hk = hdr[e.y];
     Real acc[F::NOUT];
 #pragma unroll
     for (int i = 0; i < F::NOUT; ++i) acc[i] = Real(0);
-    F::quartet(hb, hk, pp, tab, acc);
-    epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real>(make_ctx(hb, hk), ep, acc);
+    if (active) F::quartet(hb, hk, pp, tab, acc, sub, G);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < F::NOUT; ++i) acc[i] += __shfl_xor(acc[i], off, G);
+    }
+#endif
+    if (active && sub == 0) epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real>(make_ctx(hb, hk), ep, acc);
 }
+
+template <int LA, int LB, int LC, int LD, class Real>
+HFX_HD __forceinline__ void family_a_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
+                                         const EpiArgs& ep) {
+    family_a_group<LA, LB, LC, LD, Real, 1>(e, true, 0, hdr, pp, tab, ep);
+}
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+// CPU emulation of a G-lane group: the G partial sums are formed one lane at a
+// time and added, exactly the partition the device uses.
+template <int LA, int LB, int LC, int LD, class Real>
+inline void family_a_group_emu(uint2 e, int G, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
+                               const EpiArgs& ep) {
+    using F = FamA<LA, LB, LC, LD, Real>;
+    const PairHdr hb = hdr[e.x], hk = hdr[e.y];
+    Real tot[F::NOUT] = {};
+    for (int sub = 0; sub < G; ++sub) {
+        Real acc[F::NOUT] = {};
+        F::quartet(hb, hk, pp, tab, acc, sub, G);
+        for (int i = 0; i < F::NOUT; ++i) tot[i] += acc[i];
+    }
+    epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real>(make_ctx(hb, hk), ep, tot);
+}
+#endif
 
 // ------------------------------------------------------------------ family B
 // Shared-memory footprint is sized per class; the ket component pairs (cd)

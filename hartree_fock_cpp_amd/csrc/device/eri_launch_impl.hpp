@@ -13,13 +13,18 @@
 
 namespace hfx {
 
-template <int LA, int LB, int LC, int LD, class Real>
-__global__ __launch_bounds__(128) void k_fam_a(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
-                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
-                                                EpiArgs ep) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    family_a_one<LA, LB, LC, LD, Real>(list[i], hdr, pp, tab, ep);
+constexpr int FAM_A_BLOCK = 256;
+
+// G lanes per shell quartet; FAM_A_BLOCK / G quartets per workgroup.
+template <int LA, int LB, int LC, int LD, class Real, int G>
+__global__ __launch_bounds__(FAM_A_BLOCK) void k_fam_a(const uint2* __restrict__ list, int n,
+                                                        const PairHdr* __restrict__ hdr,
+                                                        const PrimPair<Real>* __restrict__ pp,
+                                                        const Real* __restrict__ tab, EpiArgs ep) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = t / G;
+    const bool active = q < n;
+    family_a_group<LA, LB, LC, LD, Real, G>(list[active ? q : n - 1], active, t % G, hdr, pp, tab, ep);
 }
 
 template <int LA, int LB, int LC, int LD, class Real>
@@ -39,9 +44,15 @@ hipError_t launch_class(const ClassLaunch& L, hipStream_t st) {
     const PrimPair<Real>* pp = static_cast<const PrimPair<Real>*>(L.prim);
     const Real* tab = static_cast<const Real*>(L.boys);
     if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
-        const int block = 128;
-        const int grid = (L.n + block - 1) / block;
-        k_fam_a<C::LA, C::LB, C::LC, C::LD, Real><<<grid, block, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep);
+        const long long lanes = (long long)L.n * L.G;
+        const int grid = (int)((lanes + FAM_A_BLOCK - 1) / FAM_A_BLOCK);
+        switch (L.G) {
+            case 1: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 1><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            case 4: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 4><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            case 16: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 16><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            case 64: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 64><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            default: return hipErrorInvalidValue;
+        }
     } else {
         const int grid = L.n < 65536 ? L.n : 65536;
         k_fam_b<C::LA, C::LB, C::LC, C::LD, Real><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.herm, L.ep);

@@ -86,6 +86,19 @@ void dfree(T*& p) {
 
 }  // namespace
 
+// Quartet lists of one plan, resident on the device, with per-class work counts.
+struct DevLists {
+    uint2* d = nullptr;
+    size_t off[NQCLASS][2] = {};
+    int cnt[NQCLASS][2] = {};
+    int bucket[NQCLASS][2][NGROUP] = {};
+    long long nprimq[NQCLASS][2] = {};   // sum over quartets of kept bra x ket primitive pairs
+    bool valid = false;
+    double threshold = -1.0;
+    int mixed = 0;
+    double split = 0.0;
+};
+
 struct hfx_basis {
     BasisPlan plan;
     PairHdr* d_hdr = nullptr;
@@ -97,8 +110,22 @@ struct hfx_basis {
     bool bounds_ready = false;
     std::vector<double> bounds;      // host copy
     std::vector<double> Qsh;         // shell-pair maxima
+    DevLists unique_plan;            // cached FP64 plan of the last unique-ERI build
     hipStream_t stream = nullptr;
 };
+
+namespace {
+
+// Optional per-class timing of ERI launches (HIP events on the launch stream).
+struct ClassTimer {
+    bool on = false;
+    hipEvent_t ev[NQCLASS][2][2] = {};
+    bool used[NQCLASS][2] = {};
+    bool created = false;
+};
+ClassTimer g_timer;
+
+}  // namespace
 
 struct hfx_fock {
     hfx_basis* b = nullptr;
@@ -107,9 +134,7 @@ struct hfx_fock {
     size_t bytes = 0;
     size_t nq = 0;
     double* d_eri = nullptr;                 // stored mode
-    uint2* d_lists = nullptr;                // direct mode: all lists back to back
-    size_t off[NQCLASS][2] = {};
-    int cnt[NQCLASS][2] = {};
+    DevLists plan;                           // direct mode: all lists back to back
     double *d_Dj = nullptr, *d_Da = nullptr, *d_Db = nullptr;
     double *d_J = nullptr, *d_Ka = nullptr, *d_Kb = nullptr;
 };
@@ -130,36 +155,70 @@ EpiArgs base_epi(const hfx_basis* b) {
     return ep;
 }
 
-int launch_lists(const hfx_basis* b, const uint2* d_lists, const size_t (*off)[2], const int (*cnt)[2],
-                 const EpiArgs& ep, hipStream_t st) {
+int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipStream_t st) {
     for (int qc = 0; qc < NQCLASS; ++qc)
         for (int p = 0; p < 2; ++p) {
-            if (cnt[qc][p] == 0) continue;
-            ClassLaunch L;
-            L.list = d_lists + off[qc][p];
-            L.n = cnt[qc][p];
-            L.hdr = b->d_hdr;
-            L.prim = p ? (const void*)b->d_ppf : (const void*)b->d_ppd;
-            L.boys = p ? (const void*)g_consts.boys_f : (const void*)g_consts.boys_d;
-            L.herm = g_consts.herm;
-            L.ep = ep;
-            HIPCHK(launch_eri_class(qc, p == 1, L, st));
+            g_timer.used[qc][p] = false;
+            if (dl.cnt[qc][p] == 0) continue;
+            if (g_timer.on) HIPCHK(hipEventRecord(g_timer.ev[qc][p][0], st));
+            size_t off = dl.off[qc][p];
+            for (int g = 0; g < NGROUP; ++g) {
+                const int n = dl.bucket[qc][p][g];
+                if (n == 0) continue;
+                ClassLaunch L;
+                L.list = dl.d + off;
+                L.n = n;
+                L.hdr = b->d_hdr;
+                L.prim = p ? (const void*)b->d_ppf : (const void*)b->d_ppd;
+                L.boys = p ? (const void*)g_consts.boys_f : (const void*)g_consts.boys_d;
+                L.herm = g_consts.herm;
+                L.G = GROUP_SIZE[g];
+                L.ep = ep;
+                HIPCHK(launch_eri_class(qc, p == 1, L, st));
+                off += n;
+            }
+            if (g_timer.on) {
+                HIPCHK(hipEventRecord(g_timer.ev[qc][p][1], st));
+                g_timer.used[qc][p] = true;
+            }
         }
     return 0;
 }
 
-// Uploads quartet lists into one device buffer; fills offsets / counts.
-int upload_lists(const QuartetLists& ql, uint2** d_lists, size_t (*off)[2], int (*cnt)[2]) {
+// Uploads quartet lists into one device buffer; fills offsets / counts / buckets.
+int upload_lists(const QuartetLists& ql, DevLists& dl) {
     std::vector<uint2> all;
     all.reserve(ql.total());
     for (int qc = 0; qc < NQCLASS; ++qc)
         for (int p = 0; p < 2; ++p) {
-            off[qc][p] = all.size();
-            cnt[qc][p] = (int)ql.list[qc][p].size();
+            dl.off[qc][p] = all.size();
+            dl.cnt[qc][p] = (int)ql.list[qc][p].size();
+            for (int g = 0; g < NGROUP; ++g) dl.bucket[qc][p][g] = ql.bucket[qc][p][g];
             all.insert(all.end(), ql.list[qc][p].begin(), ql.list[qc][p].end());
         }
-    if (int rc = dalloc(d_lists, all.size())) return rc;
-    if (!all.empty()) HIPCHK(hipMemcpy(*d_lists, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    if (int rc = dalloc(&dl.d, all.size())) return rc;
+    if (!all.empty()) HIPCHK(hipMemcpy(dl.d, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Plans the screened quartets of a basis and uploads them (cached per basis
+// for the FP64 unique build).
+int make_plan(hfx_basis* b, double thr, int mixed, double split, DevLists& dl) {
+    dfree(dl.d);
+    dl.valid = false;
+    QuartetLists ql;
+    plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql);
+    if (int rc = upload_lists(ql, dl)) return rc;
+    for (int qc = 0; qc < NQCLASS; ++qc)
+        for (int p = 0; p < 2; ++p) {
+            long long s = 0;
+            for (const uint2& e : ql.list[qc][p]) s += (long long)b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim;
+            dl.nprimq[qc][p] = s;
+        }
+    dl.threshold = thr;
+    dl.mixed = mixed;
+    dl.split = split;
+    dl.valid = true;
     return 0;
 }
 
@@ -169,23 +228,30 @@ int compute_bounds(hfx_basis* b) {
     QuartetLists ql;
     std::vector<uint2> diag[NQCLASS];
     plan_diag(b->plan, diag);
-    for (int c = 0; c < NQCLASS; ++c) ql.list[c][0] = diag[c];
-    uint2* d_lists = nullptr;
-    size_t off[NQCLASS][2];
-    int cnt[NQCLASS][2];
-    if (int rc = upload_lists(ql, &d_lists, off, cnt)) return rc;
+    for (int c = 0; c < NQCLASS; ++c) {
+        ql.list[c][0] = diag[c];
+        const bool fa = qc_family_a(c);
+        for (const uint2& e : diag[c])
+            ++ql.bucket[c][0][fa ? group_bucket(b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim) : NGROUP - 1];
+        // diag lists are built in pair order: sort them like plan_quartets does
+        std::stable_sort(ql.list[c][0].begin(), ql.list[c][0].end(), [&](const uint2& x, const uint2& y) {
+            return b->plan.hdr[x.x].nprim * b->plan.hdr[x.y].nprim > b->plan.hdr[y.x].nprim * b->plan.hdr[y.y].nprim;
+        });
+    }
+    DevLists dl;
+    if (int rc = upload_lists(ql, dl)) return rc;
     HIPCHK(launch_zero(b->d_bounds, P, b->stream));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_DIAG;
     ep.out = b->d_bounds;
-    int rc = launch_lists(b, d_lists, off, cnt, ep, b->stream);
+    int rc = launch_plan(b, dl, ep, b->stream);
     if (rc == 0) {
         b->bounds.resize(P);
         hipError_t e = hipMemcpyAsync(b->bounds.data(), b->d_bounds, P * sizeof(double), hipMemcpyDeviceToHost, b->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("schwarz bounds: ") + hipGetErrorString(e));
     }
-    dfree(d_lists);
+    dfree(dl.d);
     if (rc) return rc;
     shell_pair_bounds(b->plan, b->bounds.data(), b->Qsh);
     b->bounds_ready = true;
@@ -194,28 +260,17 @@ int compute_bounds(hfx_basis* b) {
 
 int eri_unique_into(hfx_basis* b, double threshold, double* d_out, hipStream_t st) {
     if (int rc = compute_bounds(b)) return rc;
-    QuartetLists ql;
-    plan_quartets(b->plan, b->Qsh, threshold, false, 0.0, ql);
-    uint2* d_lists = nullptr;
-    size_t off[NQCLASS][2];
-    int cnt[NQCLASS][2];
-    if (int rc = upload_lists(ql, &d_lists, off, cnt)) return rc;
-    int rc = 0;
-    hipError_t e = launch_zero(d_out, n_unique(b->plan.K), st);
-    if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("zero: ") + hipGetErrorString(e));
-    if (rc == 0) {
-        EpiArgs ep = base_epi(b);
-        ep.mode = EPI_UNIQUE;
-        ep.threshold = threshold;
-        ep.out = d_out;
-        rc = launch_lists(b, d_lists, off, cnt, ep, st);
-    }
-    if (rc == 0) {
-        e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("eri kernels: ") + hipGetErrorString(e));
-    }
-    dfree(d_lists);
-    return rc;
+    DevLists& dl = b->unique_plan;
+    if (!dl.valid || dl.threshold != threshold)
+        if (int rc = make_plan(b, threshold, 0, 0.0, dl)) return rc;
+    HIPCHK(launch_zero(d_out, n_unique(b->plan.K), st));
+    EpiArgs ep = base_epi(b);
+    ep.mode = EPI_UNIQUE;
+    ep.threshold = threshold;
+    ep.out = d_out;
+    if (int rc = launch_plan(b, dl, ep, st)) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
 }
 
 // J/K from a device unique array into device accumulators, then symmetrised.
@@ -291,6 +346,7 @@ void hfx_basis_destroy(hfx_basis* b) {
     dfree(b->d_fscale);
     dfree(b->d_bounds);
     dfree(b->d_fnpairs);
+    dfree(b->unique_plan.d);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -384,17 +440,25 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     const size_t mats = 6 * K2 * sizeof(double);
     int mode = opts->mode;
     if (opts->precision == HFX_PREC_MIXED) mode = HFX_MODE_DIRECT;   // the mixed form is a direct build
-    QuartetLists ql;
-    plan_quartets(b->plan, b->Qsh, opts->threshold, opts->precision == HFX_PREC_MIXED, f->opts.mixed_split, ql);
-    f->nq = ql.total();
+    {
+        const int mixed = opts->precision == HFX_PREC_MIXED;
+        if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split, f->plan)) {
+            hfx_fock_destroy(f);
+            return rc;
+        }
+        size_t n = 0;
+        for (int qc = 0; qc < NQCLASS; ++qc) n += f->plan.cnt[qc][0] + f->plan.cnt[qc][1];
+        f->nq = n;
+    }
     const size_t stored_bytes = Q * sizeof(double) + mats;
     const size_t direct_bytes = f->nq * sizeof(uint2) + mats;
-    if (mode == HFX_MODE_AUTO) mode = (opts->hbm_budget == 0 || stored_bytes <= opts->hbm_budget) ? HFX_MODE_STORED
-                                                                                                 : HFX_MODE_DIRECT;
+    if (mode == HFX_MODE_AUTO)
+        mode = (opts->hbm_budget == 0 || stored_bytes <= opts->hbm_budget) ? HFX_MODE_STORED : HFX_MODE_DIRECT;
     f->mode = mode;
     f->bytes = mode == HFX_MODE_STORED ? stored_bytes : direct_bytes;
+    if (mode == HFX_MODE_STORED) dfree(f->plan.d);   // stored builds use the basis' cached unique plan
     if (opts->hbm_budget && f->bytes > opts->hbm_budget) {
-        delete f;
+        hfx_fock_destroy(f);
         return fail(HFX_ERR_NOMEM, "job does not fit its HBM budget");
     }
     double* buf = nullptr;
@@ -407,15 +471,9 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
         f->d_Ka = buf + 4 * K2;
         f->d_Kb = buf + 5 * K2;
     }
-    if (!rc) {
-        for (int qc = 0; qc < NQCLASS; ++qc)
-            for (int p = 0; p < 2; ++p) f->cnt[qc][p] = (int)ql.list[qc][p].size();
-        if (mode == HFX_MODE_STORED) {
-            rc = dalloc(&f->d_eri, Q);
-            if (!rc) rc = eri_unique_into(b, opts->threshold, f->d_eri, b->stream);
-        } else {
-            rc = upload_lists(ql, &f->d_lists, f->off, f->cnt);
-        }
+    if (!rc && mode == HFX_MODE_STORED) {
+        rc = dalloc(&f->d_eri, Q);
+        if (!rc) rc = eri_unique_into(b, opts->threshold, f->d_eri, b->stream);
     }
     if (rc) {
         hfx_fock_destroy(f);
@@ -428,7 +486,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
 void hfx_fock_destroy(hfx_fock* f) {
     if (!f) return;
     dfree(f->d_eri);
-    dfree(f->d_lists);
+    dfree(f->plan.d);
     dfree(f->d_Dj);   // one allocation for all six matrices
     delete f;
 }
@@ -444,8 +502,8 @@ int hfx_fock_info(const hfx_fock* f, int* mode, size_t* device_bytes, size_t* n_
 int hfx_fock_class_counts(const hfx_fock* f, long long* n64, long long* n32) {
     if (!f) return fail(HFX_ERR_INVALID, "null fock");
     for (int qc = 0; qc < NQCLASS; ++qc) {
-        if (n64) n64[qc] = f->cnt[qc][0];
-        if (n32) n32[qc] = f->cnt[qc][1];
+        if (n64) n64[qc] = f->plan.cnt[qc][0];
+        if (n32) n32[qc] = f->plan.cnt[qc][1];
     }
     return 0;
 }
@@ -480,7 +538,7 @@ int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, d
     ep.J = d_J;
     ep.Ka = d_Ka;
     ep.Kb = d_Kb;
-    if (int rc = launch_lists(b, f->d_lists, f->off, f->cnt, ep, st)) return rc;
+    if (int rc = launch_plan(b, f->plan, ep, st)) return rc;
     HIPCHK(launch_symmetrize(d_J, K, st));
     HIPCHK(launch_symmetrize(d_Ka, K, st));
     if (nspin == 2) HIPCHK(launch_symmetrize(d_Kb, K, st));
@@ -500,6 +558,43 @@ int hfx_fock_build(hfx_fock* f, const double* Pa, const double* Pb, double* J, d
     if (Ka) HIPCHK(hipMemcpyAsync(Ka, f->d_Ka, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     if (Kb && Pb) HIPCHK(hipMemcpyAsync(Kb, f->d_Kb, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hfx_set_class_timing(int on) {
+    if (on && !g_timer.created) {
+        for (int qc = 0; qc < NQCLASS; ++qc)
+            for (int p = 0; p < 2; ++p)
+                for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&g_timer.ev[qc][p][k]));
+        g_timer.created = true;
+    }
+    g_timer.on = on != 0;
+    return 0;
+}
+
+int hfx_class_times(float* ms) {
+    for (int qc = 0; qc < NQCLASS; ++qc)
+        for (int p = 0; p < 2; ++p) {
+            float t = 0.0f;
+            if (g_timer.used[qc][p]) HIPCHK(hipEventElapsedTime(&t, g_timer.ev[qc][p][0], g_timer.ev[qc][p][1]));
+            ms[2 * qc + p] = t;
+        }
+    return 0;
+}
+
+int hfx_plan_stats(hfx_basis* b, double threshold, int precision, double mixed_split, long long* nq,
+                   long long* nprimq) {
+    if (!b) return fail(HFX_ERR_INVALID, "null basis");
+    if (int rc = compute_bounds(b)) return rc;
+    QuartetLists ql;
+    plan_quartets(b->plan, b->Qsh, threshold, precision == HFX_PREC_MIXED, mixed_split > 0 ? mixed_split : 1e-5, ql);
+    for (int qc = 0; qc < NQCLASS; ++qc)
+        for (int p = 0; p < 2; ++p) {
+            long long s = 0;
+            for (const uint2& e : ql.list[qc][p]) s += (long long)b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim;
+            if (nq) nq[2 * qc + p] = (long long)ql.list[qc][p].size();
+            if (nprimq) nprimq[2 * qc + p] = s;
+        }
     return 0;
 }
 

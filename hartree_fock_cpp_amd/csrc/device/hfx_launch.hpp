@@ -13,6 +13,7 @@ struct ClassLaunch {
     const void* prim;      // device PrimPair<Real> array
     const void* boys;      // device Boys table (Real)
     const HermTab* herm;   // device Hermite (t,u,v) decode table
+    int G;                 // family A lanes per quartet (1, 4, 16, 64)
     EpiArgs ep;
 };
 

@@ -181,8 +181,22 @@ inline void shell_pair_bounds(const BasisPlan& bp, const double* fnb, std::vecto
     }
 }
 
+// Lane-group sizes of family A: a quartet with w = (bra x ket kept primitive
+// pairs) runs on G lanes, G = 64 for w >= 512, 16 for w >= 64, 4 for w >= 8,
+// else 1.  Lists are sorted by w descending, so the buckets are contiguous
+// ranges [G=64 | G=16 | G=4 | G=1].
+constexpr int NGROUP = 4;
+constexpr int GROUP_SIZE[NGROUP] = {64, 16, 4, 1};
+inline int group_bucket(int w) { return w >= 512 ? 0 : w >= 64 ? 1 : w >= 8 ? 2 : 3; }
+
+inline bool qc_family_a(int qc) {
+    const int pb = qc_bra(qc), pk = qc_ket(qc);
+    return use_family_a(pc_la(pb), pc_lb(pb), pc_la(pk), pc_lb(pk));
+}
+
 struct QuartetLists {
     std::vector<uint2> list[NQCLASS][2];   // [class][0 = FP64, 1 = FP32]
+    int bucket[NQCLASS][2][NGROUP] = {};   // quartets per lane-group bucket (family A; family B uses bucket 3)
     size_t total() const {
         size_t n = 0;
         for (int c = 0; c < NQCLASS; ++c) n += list[c][0].size() + list[c][1].size();
@@ -222,6 +236,10 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
                 const int wa = bp.hdr[a.x].nprim * bp.hdr[a.y].nprim, wb = bp.hdr[b.x].nprim * bp.hdr[b.y].nprim;
                 return wa > wb;
             });
+            for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = 0;
+            const bool fa = qc_family_a(c);
+            for (const uint2& e : v)
+                ++ql.bucket[c][p][fa ? group_bucket(bp.hdr[e.x].nprim * bp.hdr[e.y].nprim) : NGROUP - 1];
         }
 }
 

@@ -121,6 +121,18 @@ int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, d
  * ss,ps,pp,ds,dp,dd,fs,fp,fd,ff), the number of FP64 and FP32 shell quartets. */
 int hfx_fock_class_counts(const hfx_fock* f, long long* n_fp64, long long* n_fp32);
 
+/* Measurement hooks (no reference counterpart).  With class timing on, every
+ * ERI class launch is bracketed by HIP events on its stream; hfx_class_times
+ * returns the last build's per-launch milliseconds, ms[2*qc + p] with p = 0
+ * for FP64 and 1 for FP32 (0 for classes not launched). */
+int hfx_set_class_timing(int on);
+int hfx_class_times(float* ms);
+
+/* Screened shell-quartet counts and primitive-quartet work (sum over quartets
+ * of kept bra x ket primitive pairs) per class, nq[2*qc + p], nprimq[2*qc + p]. */
+int hfx_plan_stats(hfx_basis* b, double threshold, int precision, double mixed_split, long long* nq,
+                   long long* nprimq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,7 +28,7 @@ struct EmuCtx {
 };
 
 template <int QC, class Real>
-void run_class(const EmuCtx& c, const std::vector<uint2>& list, const EpiArgs& ep) {
+void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucket, const EpiArgs& ep) {
     using C = QClass<QC>;
     const PrimPair<Real>* pp;
     const Real* tab;
@@ -40,7 +40,12 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const EpiArgs& e
         tab = c.boysf.data();
     }
     if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
-        for (const uint2& e : list) family_a_one<C::LA, C::LB, C::LC, C::LD, Real>(e, c.plan.hdr.data(), pp, tab, ep);
+        // replay the device's lane-group partition bucket by bucket
+        size_t pos = 0;
+        for (int g = 0; g < NGROUP; ++g)
+            for (int k = 0; k < bucket[g]; ++k, ++pos)
+                family_a_group_emu<C::LA, C::LB, C::LC, C::LD, Real>(list[pos], GROUP_SIZE[g], c.plan.hdr.data(), pp,
+                                                                     tab, ep);
     } else {
         auto sm = std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
         auto sync = []() {};
@@ -49,7 +54,7 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const EpiArgs& e
     }
 }
 
-using RunFn = void (*)(const EmuCtx&, const std::vector<uint2>&, const EpiArgs&);
+using RunFn = void (*)(const EmuCtx&, const std::vector<uint2>&, const int*, const EpiArgs&);
 
 template <class Real, int... QC>
 RunFn pick(std::integer_sequence<int, QC...>, int qc) {
@@ -59,8 +64,10 @@ RunFn pick(std::integer_sequence<int, QC...>, int qc) {
 
 void run_lists(const EmuCtx& c, const QuartetLists& ql, const EpiArgs& ep) {
     for (int qc = 0; qc < NQCLASS; ++qc) {
-        if (!ql.list[qc][0].empty()) pick<double>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][0], ep);
-        if (!ql.list[qc][1].empty()) pick<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][1], ep);
+        if (!ql.list[qc][0].empty())
+            pick<double>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][0], ql.bucket[qc][0], ep);
+        if (!ql.list[qc][1].empty())
+            pick<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][1], ql.bucket[qc][1], ep);
     }
 }
 
@@ -84,7 +91,10 @@ int make_ctx(int K, const double* xyz, const int* lmn, const int* poff, const do
     std::vector<uint2> diag[NQCLASS];
     plan_diag(c.plan, diag);
     QuartetLists ql;
-    for (int q = 0; q < NQCLASS; ++q) ql.list[q][0] = diag[q];
+    for (int q = 0; q < NQCLASS; ++q) {
+        ql.list[q][0] = diag[q];
+        ql.bucket[q][0][NGROUP - 1] = (int)diag[q].size();   // one lane per diagonal quartet
+    }
     std::vector<double> out(P, 0.0);
     EpiArgs ep = base_epi(c);
     ep.mode = EPI_DIAG;
