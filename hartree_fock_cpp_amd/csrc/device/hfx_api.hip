@@ -86,6 +86,10 @@ void dfree(T*& p) {
 
 }  // namespace
 
+namespace hfx_detail {
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace hfx_detail
+
 // Quartet lists of one plan, resident on the device, with per-class work counts.
 struct DevLists {
     uint2* d = nullptr;
