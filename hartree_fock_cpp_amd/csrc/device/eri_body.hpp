@@ -48,7 +48,7 @@ HFX_HD __forceinline__ void static_for(F&& f) {
 __host__ __device__ constexpr bool use_family_a(int la, int lb, int lc, int ld) {
     int L = la + lb + lc + ld;
     int nout = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-    return L <= 4 && nout + nherm(L) <= 72;
+    return L <= 4 && nout + nherm(L) <= 128;
 }
 
 // ------------------------------------------------------------------ epilogue
@@ -217,10 +217,11 @@ struct FamA {
 };
 
 // Family-A epilogue over a register block acc[NOUT] (raw, unscaled).
-template <int NA, int NB, int NC, int ND, class Real>
+template <int NA, int NB, int NC, int ND, class Real, bool JK>
 HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep, const Real* acc) {
     constexpr int NCD = NC * ND;
     const double* fs = ep.fscale;
+    if constexpr (!JK) {
     if (ep.mode == EPI_UNIQUE) {
 #pragma unroll
         for (int e = 0; e < NA * NB * NCD; ++e) {
@@ -247,6 +248,7 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
             }
         return;
     }
+    } else {
     // EPI_JK
     const int K = ep.K;
     const double f = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
@@ -329,6 +331,7 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
                 atomic_add(&Kacc[(q.fn2 + b) * K + q.fn4 + d], f * s);
             }
     }
+    }
 }
 
 HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk) {
@@ -343,7 +346,7 @@ HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk)
 // lane of the group must call this (inactive groups pass active = false) so
 // the cross-lane reduction sees the whole group.  Lane 0 of the group runs the
 // epilogue.
-template <int LA, int LB, int LC, int LD, class Real, int G>
+template <int LA, int LB, int LC, int LD, class Real, int G, bool JK>
 HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const PairHdr* hdr, const PrimPair<Real>* pp,
                                            const Real* tab, const EpiArgs& ep) {
     using F = FamA<LA, LB, LC, LD, Real>;
@@ -360,13 +363,16 @@ HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const 
             for (int i = 0; i < F::NOUT; ++i) acc[i] += __shfl_xor(acc[i], off, G);
     }
 #endif
-    if (active && sub == 0) epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real>(make_ctx(hb, hk), ep, acc);
+    if (active && sub == 0) epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, JK>(make_ctx(hb, hk), ep, acc);
 }
 
 template <int LA, int LB, int LC, int LD, class Real>
 HFX_HD __forceinline__ void family_a_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
                                          const EpiArgs& ep) {
-    family_a_group<LA, LB, LC, LD, Real, 1>(e, true, 0, hdr, pp, tab, ep);
+    if (ep.mode == EPI_JK)
+        family_a_group<LA, LB, LC, LD, Real, 1, true>(e, true, 0, hdr, pp, tab, ep);
+    else
+        family_a_group<LA, LB, LC, LD, Real, 1, false>(e, true, 0, hdr, pp, tab, ep);
 }
 
 #if !defined(__HIP_DEVICE_COMPILE__)
@@ -383,7 +389,10 @@ inline void family_a_group_emu(uint2 e, int G, const PairHdr* hdr, const PrimPai
         F::quartet(hb, hk, pp, tab, acc, sub, G);
         for (int i = 0; i < F::NOUT; ++i) tot[i] += acc[i];
     }
-    epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real>(make_ctx(hb, hk), ep, tot);
+    if (ep.mode == EPI_JK)
+        epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, true>(make_ctx(hb, hk), ep, tot);
+    else
+        epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, false>(make_ctx(hb, hk), ep, tot);
 }
 #endif
 
@@ -396,7 +405,8 @@ struct FamBShape {
     static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
     static constexpr int NAB = NA * NB, NCD = NC * ND;
     static constexpr int NHAB = nherm(LAB), NR = nherm(L);
-    static constexpr int BUDGET = 5600;   // doubles for g + acc per chunk
+    static constexpr int KB = L <= 8 ? 4 : 1;   // ket primitive pairs processed per step
+    static constexpr int BUDGET = 5600 - (KB - 1) * 2 * NR;   // doubles for g + acc per chunk
     static constexpr int CDC0 = BUDGET / (NHAB + NAB);
     static constexpr int CDC = CDC0 < NCD ? CDC0 : NCD;
     static constexpr int NCHUNK = (NCD + CDC - 1) / CDC;
@@ -404,25 +414,63 @@ struct FamBShape {
     static constexpr int EK = 3 * (LC + 1) * (LD + 1) * (LCD + 1);
 };
 
-template <int LA, int LB, int LC, int LD, class Real>
-struct FamBSmem {
-    using S = FamBShape<LA, LB, LC, LD>;
-    Real R[2][S::NR];
-    Real g[S::NHAB * S::CDC];
-    Real acc[S::NAB * S::CDC];
-    Real eb[S::EB];
-    Real ek[S::EK];
-    Real a[S::L + 1];
-    Real rx[3];
-};
-
 // Hermite triple tables for the runtime-indexed loops of family B.
 struct HermTab {
     unsigned char t[455], u[455], v[455];
 };
 
+template <int LA, int LB, int LC, int LD, class Real>
+struct FamBSmem {
+    using S = FamBShape<LA, LB, LC, LD>;
+    Real R[S::KB][2][S::NR];
+    Real g[S::NHAB * S::CDC];
+    Real acc[S::NAB * S::CDC];
+    Real eb[S::EB];
+    Real ek[S::KB][S::EK];
+    Real a[S::KB][S::L + 1];
+    Real rx[S::KB][3];
+    // class-constant decode tables, filled once per workgroup
+    unsigned char ht[S::NR][3];        // packed Hermite index -> (t, u, v)
+    unsigned char abc[S::NAB][6];      // bra component pair -> (ax, ay, az, bx, by, bz)
+    unsigned char cdc[S::NCD][6];      // ket component pair -> (cx, cy, cz, dx, dy, dz)
+};
+
+// Fills the decode tables of a family-B workgroup (lanes stride the entries).
+template <int LA, int LB, int LC, int LD, class Real>
+HFX_HD __forceinline__ void famb_tables(FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht, int lane, int nlanes) {
+    using S = FamBShape<LA, LB, LC, LD>;
+    for (int w = lane; w < S::NR; w += nlanes) {
+        sm.ht[w][0] = ht.t[w];
+        sm.ht[w][1] = ht.u[w];
+        sm.ht[w][2] = ht.v[w];
+    }
+    for (int w = lane; w < S::NAB; w += nlanes) {
+        const int a = w / S::NB, b = w % S::NB;
+        sm.abc[w][0] = (unsigned char)cart_x(LA, a);
+        sm.abc[w][1] = (unsigned char)cart_y(LA, a);
+        sm.abc[w][2] = (unsigned char)cart_z(LA, a);
+        sm.abc[w][3] = (unsigned char)cart_x(LB, b);
+        sm.abc[w][4] = (unsigned char)cart_y(LB, b);
+        sm.abc[w][5] = (unsigned char)cart_z(LB, b);
+    }
+    for (int w = lane; w < S::NCD; w += nlanes) {
+        const int c = w / S::ND, d = w % S::ND;
+        sm.cdc[w][0] = (unsigned char)cart_x(LC, c);
+        sm.cdc[w][1] = (unsigned char)cart_y(LC, c);
+        sm.cdc[w][2] = (unsigned char)cart_z(LC, c);
+        sm.cdc[w][3] = (unsigned char)cart_x(LD, d);
+        sm.cdc[w][4] = (unsigned char)cart_y(LD, d);
+        sm.cdc[w][5] = (unsigned char)cart_z(LD, d);
+    }
+}
+
+// 1-D E tables (x, y, z) of a primitive pair into LDS, odd Hermite orders
+// negated for the ket.  All indices are compile-time constants so the pair
+// stays in registers; one lane per group runs it (`lane == 0`).
 template <int LA, int LB, class Real>
-HFX_HD __forceinline__ void build_e_to_smem(Real* dst, const PrimPair<Real>& pp, bool sign) {
+HFX_HD __forceinline__ void build_e_lanes(Real* dst, const PrimPair<Real>& pp, bool sign, int lane, int nlanes) {
+    if (lane != 0) return;
+    (void)nlanes;
     static_for<0, 3>([&](auto xi) {
         constexpr int x = decltype(xi)::value;
         Herm1D<LA, LB, Real> h;
@@ -440,7 +488,7 @@ HFX_HD __forceinline__ void build_e_to_smem(Real* dst, const PrimPair<Real>& pp,
 
 // One wavefront (or one CPU thread with nlanes = 1) computes one shell quartet.
 // `sync()` must be a wavefront/workgroup barrier on the device.
-template <int LA, int LB, int LC, int LD, class Real, class Sync>
+template <int LA, int LB, int LC, int LD, class Real, bool JK, class Sync>
 HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab, const EpiArgs& ep,
                          FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht, int lane, int nlanes, Sync sync) {
     using S = FamBShape<LA, LB, LC, LD>;
@@ -451,6 +499,7 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
     const double* fs = ep.fscale;
     const int K = ep.K;
     const double fdeg = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
+    famb_tables<LA, LB, LC, LD, Real>(sm, ht, lane, nlanes);
 
     for (int ch = 0; ch < S::NCHUNK; ++ch) {
         const int cd0 = ch * S::CDC;
@@ -459,77 +508,97 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
         for (int i = 0; i < hb.nprim; ++i) {
             const PrimPair<Real> b = pp[hb.prim_off + i];
             sync();
-            if (lane == 0) build_e_to_smem<LA, LB, Real>(sm.eb, b, false);
+            build_e_lanes<LA, LB, Real>(sm.eb, b, false, lane, nlanes);
             for (int w = lane; w < S::NHAB * S::CDC; w += nlanes) sm.g[w] = Real(0);
-            for (int j = 0; j < hk.nprim; ++j) {
-                const PrimPair<Real> k = pp[hk.prim_off + j];
+            // KB ket primitive pairs per step: on the device the wavefront splits
+            // into KB groups of grp lanes, one pair each; with one lane the
+            // groups run one after another.
+            const int grp = nlanes >= S::KB ? nlanes / S::KB : 1;
+            const int kb0 = nlanes >= S::KB ? lane / grp : 0;
+            const int kbstep = nlanes >= S::KB ? S::KB : 1;
+            const int gl = nlanes >= S::KB ? lane % grp : 0;
+            for (int j0 = 0; j0 < hk.nprim; j0 += S::KB) {
+                const int nkb = (hk.nprim - j0) < S::KB ? (hk.nprim - j0) : S::KB;
                 sync();
-                if (lane == 0) build_e_to_smem<LC, LD, Real>(sm.ek, k, true);
-                if (lane == 1 || nlanes == 1) {
-                    Real s = b.p + k.p;
-                    Real inv = Real(1) / s;
-                    Real rho = b.p * k.p * inv;
-                    Real X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
-                    Real T = rho * (X * X + Y * Y + Z * Z);
-                    Real F[S::L + 1];
-                    boys_fm<S::L, Real>(T, tab, F);
-                    Real f = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
-                    Real m2r = Real(-2) * rho;
-                    for (int n = 0; n <= S::L; ++n) {
-                        sm.a[n] = f * F[n];
-                        f *= m2r;
+                for (int kb = kb0; kb < nkb; kb += kbstep) {
+                    const PrimPair<Real> k = pp[hk.prim_off + j0 + kb];
+                    build_e_lanes<LC, LD, Real>(sm.ek[kb], k, true, gl, grp);
+                    if (gl == 1 || grp == 1) {
+                        Real s = b.p + k.p;
+                        Real inv = Real(1) / s;
+                        Real rho = b.p * k.p * inv;
+                        Real X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
+                        Real T = rho * (X * X + Y * Y + Z * Z);
+                        Real F[S::L + 1];
+                        boys_fm<S::L, Real>(T, tab, F);
+                        Real f = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
+                        Real m2r = Real(-2) * rho;
+                        for (int n = 0; n <= S::L; ++n) {
+                            sm.a[kb][n] = f * F[n];
+                            f *= m2r;
+                        }
+                        sm.rx[kb][0] = X;
+                        sm.rx[kb][1] = Y;
+                        sm.rx[kb][2] = Z;
                     }
-                    sm.rx[0] = X; sm.rx[1] = Y; sm.rx[2] = Z;
                 }
                 sync();
                 // R levels n = L .. 0 : level n holds orders <= L - n, ping-pong buffers
+                for (int kb = kb0; kb < nkb; kb += kbstep)
+                    if (gl == 0) sm.R[kb][0][0] = sm.a[kb][S::L];
                 int cur = 0;
-                if (lane == 0) sm.R[0][0] = sm.a[S::L];
                 for (int n = S::L - 1; n >= 0; --n) {
                     sync();
-                    const Real* Rp = sm.R[cur];
-                    Real* Rn = sm.R[cur ^ 1];
                     const int cnt = nherm(S::L - n);
-                    for (int w = lane; w < cnt; w += nlanes) {
-                        const int t = ht.t[w], u = ht.u[w], v = ht.v[w];
-                        Real val;
-                        if (t > 0) {
-                            val = sm.rx[0] * Rp[hidx(t - 1, u, v)];
-                            if (t > 1) val += Real(t - 1) * Rp[hidx(t - 2, u, v)];
-                        } else if (u > 0) {
-                            val = sm.rx[1] * Rp[hidx(t, u - 1, v)];
-                            if (u > 1) val += Real(u - 1) * Rp[hidx(t, u - 2, v)];
-                        } else if (v > 0) {
-                            val = sm.rx[2] * Rp[hidx(t, u, v - 1)];
-                            if (v > 1) val += Real(v - 1) * Rp[hidx(t, u, v - 2)];
-                        } else {
-                            val = sm.a[n];
+                    for (int kb = kb0; kb < nkb; kb += kbstep) {
+                        const Real* Rp = sm.R[kb][cur];
+                        Real* Rn = sm.R[kb][cur ^ 1];
+                        const Real rx = sm.rx[kb][0], ry = sm.rx[kb][1], rz = sm.rx[kb][2];
+                        for (int w = gl; w < cnt; w += grp) {
+                            const int t = sm.ht[w][0], u = sm.ht[w][1], v = sm.ht[w][2];
+                            Real val;
+                            if (t > 0) {
+                                val = rx * Rp[hidx(t - 1, u, v)];
+                                if (t > 1) val += Real(t - 1) * Rp[hidx(t - 2, u, v)];
+                            } else if (u > 0) {
+                                val = ry * Rp[hidx(t, u - 1, v)];
+                                if (u > 1) val += Real(u - 1) * Rp[hidx(t, u - 2, v)];
+                            } else if (v > 0) {
+                                val = rz * Rp[hidx(t, u, v - 1)];
+                                if (v > 1) val += Real(v - 1) * Rp[hidx(t, u, v - 2)];
+                            } else {
+                                val = sm.a[kb][n];
+                            }
+                            Rn[w] = val;
                         }
-                        Rn[w] = val;
                     }
                     cur ^= 1;
                 }
                 sync();
-                // stage 1: g[tuv][cd] += sum Eket_cd(tau nu phi) R(t+tau, u+nu, v+phi)
-                const Real* R = sm.R[cur];
+                // stage 1: g[tuv][cd] += sum_kb sum Eket_cd(tau nu phi) R_kb(t+tau, u+nu, v+phi)
                 for (int w = lane; w < S::NHAB * ncd; w += nlanes) {
                     const int h = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
-                    const int c = cd / S::ND, d = cd % S::ND;
-                    const int t = ht.t[h], u = ht.u[h], v = ht.v[h];
-                    const int cx = cart_x(LC, c), cy = cart_y(LC, c), cz = cart_z(LC, c);
-                    const int dx = cart_x(LD, d), dy = cart_y(LD, d), dz = cart_z(LD, d);
-                    const Real* Ex = sm.ek + (cx * (LD + 1) + dx) * (S::LCD + 1);
-                    const Real* Ey = sm.ek + SK + (cy * (LD + 1) + dy) * (S::LCD + 1);
-                    const Real* Ez = sm.ek + 2 * SK + (cz * (LD + 1) + dz) * (S::LCD + 1);
+                    const int t = sm.ht[h][0], u = sm.ht[h][1], v = sm.ht[h][2];
+                    const int cx = sm.cdc[cd][0], cy = sm.cdc[cd][1], cz = sm.cdc[cd][2];
+                    const int dx = sm.cdc[cd][3], dy = sm.cdc[cd][4], dz = sm.cdc[cd][5];
+                    const int ox = (cx * (LD + 1) + dx) * (S::LCD + 1);
+                    const int oy = SK + (cy * (LD + 1) + dy) * (S::LCD + 1);
+                    const int oz = 2 * SK + (cz * (LD + 1) + dz) * (S::LCD + 1);
                     Real s = Real(0);
-                    for (int ta = 0; ta <= cx + dx; ++ta) {
-                        Real sy = Real(0);
-                        for (int nn = 0; nn <= cy + dy; ++nn) {
-                            Real sz = Real(0);
-                            for (int ph = 0; ph <= cz + dz; ++ph) sz += Ez[ph] * R[hidx(t + ta, u + nn, v + ph)];
-                            sy += Ey[nn] * sz;
+                    for (int kb = 0; kb < nkb; ++kb) {
+                        const Real* R = sm.R[kb][cur];
+                        const Real* Ex = sm.ek[kb] + ox;
+                        const Real* Ey = sm.ek[kb] + oy;
+                        const Real* Ez = sm.ek[kb] + oz;
+                        for (int ta = 0; ta <= cx + dx; ++ta) {
+                            Real sy = Real(0);
+                            for (int nn = 0; nn <= cy + dy; ++nn) {
+                                Real sz = Real(0);
+                                for (int ph = 0; ph <= cz + dz; ++ph) sz += Ez[ph] * R[hidx(t + ta, u + nn, v + ph)];
+                                sy += Ey[nn] * sz;
+                            }
+                            s += Ex[ta] * sy;
                         }
-                        s += Ex[ta] * sy;
                     }
                     sm.g[h * S::CDC + cdl] += s;
                 }
@@ -538,9 +607,8 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
             // stage 2: acc[ab][cd] += sum_tuv Ebra_ab(tuv) g[tuv][cd]
             for (int w = lane; w < S::NAB * ncd; w += nlanes) {
                 const int ab = w / ncd, cdl = w % ncd;
-                const int a = ab / S::NB, bb = ab % S::NB;
-                const int ax = cart_x(LA, a), ay = cart_y(LA, a), az = cart_z(LA, a);
-                const int bx = cart_x(LB, bb), by = cart_y(LB, bb), bz = cart_z(LB, bb);
+                const int ax = sm.abc[ab][0], ay = sm.abc[ab][1], az = sm.abc[ab][2];
+                const int bx = sm.abc[ab][3], by = sm.abc[ab][4], bz = sm.abc[ab][5];
                 const Real* Ex = sm.eb + (ax * (LB + 1) + bx) * (S::LAB + 1);
                 const Real* Ey = sm.eb + SB + (ay * (LB + 1) + by) * (S::LAB + 1);
                 const Real* Ez = sm.eb + 2 * SB + (az * (LB + 1) + bz) * (S::LAB + 1);
@@ -564,7 +632,7 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
             const int mu = q.fn1 + a, nu = q.fn2 + bb, la = q.fn3 + c, si = q.fn4 + d;
             return (double)sm.acc[ab * S::CDC + cdl] * (fs[mu] * fs[nu] * fs[la] * fs[si]);
         };
-        if (ep.mode == EPI_UNIQUE) {
+        if (!JK && ep.mode == EPI_UNIQUE) {
             for (int w = lane; w < S::NAB * ncd; w += nlanes) {
                 const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
                 const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;
@@ -574,7 +642,7 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
                 if (ep.bounds[IJ] * ep.bounds[KL] < ep.threshold) continue;
                 ep.out[quartet_of(IJ, KL)] = val(ab, cdl);
             }
-        } else if (ep.mode == EPI_DIAG) {
+        } else if (!JK && ep.mode == EPI_DIAG) {
             for (int w = lane; w < S::NAB; w += nlanes) {
                 const int cd = w;   // (ab|ab): same component pair on both sides
                 if (cd < cd0 || cd >= cd0 + ncd) continue;
@@ -583,7 +651,7 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
                 double v = val(w, cd - cd0);
                 ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
             }
-        } else {
+        } else if constexpr (JK) {
             auto mv = [&](int ab, int cdl) -> double {
                 const int cd = cd0 + cdl;
                 const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;

@@ -16,7 +16,7 @@ namespace hfx {
 constexpr int FAM_A_BLOCK = 256;
 
 // G lanes per shell quartet; FAM_A_BLOCK / G quartets per workgroup.
-template <int LA, int LB, int LC, int LD, class Real, int G>
+template <int LA, int LB, int LC, int LD, class Real, int G, bool JK>
 __global__ __launch_bounds__(FAM_A_BLOCK) void k_fam_a(const uint2* __restrict__ list, int n,
                                                         const PairHdr* __restrict__ hdr,
                                                         const PrimPair<Real>* __restrict__ pp,
@@ -24,40 +24,47 @@ __global__ __launch_bounds__(FAM_A_BLOCK) void k_fam_a(const uint2* __restrict__
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = t / G;
     const bool active = q < n;
-    family_a_group<LA, LB, LC, LD, Real, G>(list[active ? q : n - 1], active, t % G, hdr, pp, tab, ep);
+    family_a_group<LA, LB, LC, LD, Real, G, JK>(list[active ? q : n - 1], active, t % G, hdr, pp, tab, ep);
 }
 
-template <int LA, int LB, int LC, int LD, class Real>
+template <int LA, int LB, int LC, int LD, class Real, bool JK>
 __global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
                                                const HermTab* __restrict__ ht, EpiArgs ep) {
     __shared__ FamBSmem<LA, LB, LC, LD, Real> sm;
     auto sync = []() { __syncthreads(); };
     for (int q = blockIdx.x; q < n; q += gridDim.x)
-        family_b_one<LA, LB, LC, LD, Real>(list[q], hdr, pp, tab, ep, sm, *ht, (int)threadIdx.x, 64, sync);
+        family_b_one<LA, LB, LC, LD, Real, JK>(list[q], hdr, pp, tab, ep, sm, *ht, (int)threadIdx.x, 64, sync);
 }
 
-template <int QC, class Real>
-hipError_t launch_class(const ClassLaunch& L, hipStream_t st) {
+template <int QC, class Real, bool JK>
+hipError_t launch_class_mode(const ClassLaunch& L, hipStream_t st) {
     using C = QClass<QC>;
-    if (L.n <= 0) return hipSuccess;
     const PrimPair<Real>* pp = static_cast<const PrimPair<Real>*>(L.prim);
     const Real* tab = static_cast<const Real*>(L.boys);
     if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
         const long long lanes = (long long)L.n * L.G;
         const int grid = (int)((lanes + FAM_A_BLOCK - 1) / FAM_A_BLOCK);
         switch (L.G) {
-            case 1: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 1><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
-            case 4: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 4><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
-            case 16: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 16><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
-            case 64: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 64><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            case 1: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 1, JK><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            case 4: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 4, JK><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            case 16: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 16, JK><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
+            case 64: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 64, JK><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
             default: return hipErrorInvalidValue;
         }
     } else {
         const int grid = L.n < 65536 ? L.n : 65536;
-        k_fam_b<C::LA, C::LB, C::LC, C::LD, Real><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.herm, L.ep);
+        k_fam_b<C::LA, C::LB, C::LC, C::LD, Real, JK><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.herm, L.ep);
     }
     return hipGetLastError();
+}
+
+// The write epilogues (Schwarz diagonal, unique array) and the J/K epilogue
+// are separate instantiations so the write kernels keep a small register file.
+template <int QC, class Real>
+hipError_t launch_class(const ClassLaunch& L, hipStream_t st) {
+    if (L.n <= 0) return hipSuccess;
+    return L.ep.mode == EPI_JK ? launch_class_mode<QC, Real, true>(L, st) : launch_class_mode<QC, Real, false>(L, st);
 }
 
 template <int Begin, int... I>

@@ -1,3 +1,3 @@
-// eri_part1.hip -- instantiates the ERI kernels of classes [15, 28).
+// eri_part1.hip -- instantiates the ERI kernels of classes [4, 7).
 #include "eri_launch_impl.hpp"
-HFX_DEFINE_PART(launch_eri_part1, 15, 28)
+HFX_DEFINE_PART(launch_eri_part1, 4, 7)

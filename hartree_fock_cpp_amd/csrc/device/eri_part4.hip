@@ -1,3 +1,3 @@
-// eri_part4.hip -- instantiates the ERI kernels of classes [47, 55).
+// eri_part4.hip -- instantiates the ERI kernels of classes [13, 21).
 #include "eri_launch_impl.hpp"
-HFX_DEFINE_PART(launch_eri_part4, 47, 55)
+HFX_DEFINE_PART(launch_eri_part4, 13, 21)

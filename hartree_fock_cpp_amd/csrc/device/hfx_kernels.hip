@@ -20,15 +20,21 @@ hipError_t launch_eri_part1(int, bool, const ClassLaunch&, hipStream_t);
 hipError_t launch_eri_part2(int, bool, const ClassLaunch&, hipStream_t);
 hipError_t launch_eri_part3(int, bool, const ClassLaunch&, hipStream_t);
 hipError_t launch_eri_part4(int, bool, const ClassLaunch&, hipStream_t);
+hipError_t launch_eri_part5(int, bool, const ClassLaunch&, hipStream_t);
+hipError_t launch_eri_part6(int, bool, const ClassLaunch&, hipStream_t);
+hipError_t launch_eri_part7(int, bool, const ClassLaunch&, hipStream_t);
 
-// Class ranges instantiated by eri_part0..4.hip.
+// Class ranges instantiated by eri_part0..7.hip.
 hipError_t launch_eri_class(int qc, bool fp32, const ClassLaunch& L, hipStream_t st) {
     if (qc < 0 || qc >= NQCLASS) return hipErrorInvalidValue;
-    if (qc < 15) return launch_eri_part0(qc, fp32, L, st);
-    if (qc < 28) return launch_eri_part1(qc, fp32, L, st);
-    if (qc < 38) return launch_eri_part2(qc, fp32, L, st);
-    if (qc < 47) return launch_eri_part3(qc, fp32, L, st);
-    return launch_eri_part4(qc, fp32, L, st);
+    if (qc < 4) return launch_eri_part0(qc, fp32, L, st);
+    if (qc < 7) return launch_eri_part1(qc, fp32, L, st);
+    if (qc < 10) return launch_eri_part2(qc, fp32, L, st);
+    if (qc < 13) return launch_eri_part3(qc, fp32, L, st);
+    if (qc < 21) return launch_eri_part4(qc, fp32, L, st);
+    if (qc < 30) return launch_eri_part5(qc, fp32, L, st);
+    if (qc < 42) return launch_eri_part6(qc, fp32, L, st);
+    return launch_eri_part7(qc, fp32, L, st);
 }
 
 bool class_uses_family_a(int qc) {

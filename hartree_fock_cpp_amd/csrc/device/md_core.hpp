@@ -52,12 +52,11 @@ __host__ __device__ constexpr int cart_y(int l, int k) {
 __host__ __device__ constexpr int cart_z(int l, int k) { return l - cart_x(l, k) - cart_y(l, k); }
 
 // Packed Hermite index of (t,u,v), t+u+v = s: all orders below s first, then
-// t ascending, then u ascending (same packing as integrals.hpp:343-358).
+// t ascending, then u ascending (same packing as integrals.hpp:343-358):
+// s(s+1)(s+2)/6 + sum_{a<t} (s-a+1) + u, in closed form.
 __host__ __device__ constexpr int hidx(int t, int u, int v) {
-    int s = t + u + v;
-    int off = s * (s + 1) * (s + 2) / 6;
-    for (int a = 0; a < t; ++a) off += (s - a + 1);
-    return off + u;
+    const int s = t + u + v;
+    return s * (s + 1) * (s + 2) / 6 + t * (s + 1) - t * (t - 1) / 2 + u;
 }
 
 // ------------------------------------------------------------------ Boys

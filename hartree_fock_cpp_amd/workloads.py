@@ -130,4 +130,4 @@ def kernel_family(qc):
     a, b, c, d = qclass_l(qc)
     L = a + b + c + d
     nout = len(_cart(a)) * len(_cart(b)) * len(_cart(c)) * len(_cart(d))
-    return "k_fam_a" if (L <= 4 and nout + math.comb(L + 3, 3) <= 72) else "k_fam_b"
+    return "k_fam_a" if (L <= 4 and nout + math.comb(L + 3, 3) <= 128) else "k_fam_b"

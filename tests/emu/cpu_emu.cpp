@@ -49,8 +49,12 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucke
     } else {
         auto sm = std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
         auto sync = []() {};
-        for (const uint2& e : list)
-            family_b_one<C::LA, C::LB, C::LC, C::LD, Real>(e, c.plan.hdr.data(), pp, tab, ep, *sm, c.ht, 0, 1, sync);
+        for (const uint2& e : list) {
+            if (ep.mode == EPI_JK)
+                family_b_one<C::LA, C::LB, C::LC, C::LD, Real, true>(e, c.plan.hdr.data(), pp, tab, ep, *sm, c.ht, 0, 1, sync);
+            else
+                family_b_one<C::LA, C::LB, C::LC, C::LD, Real, false>(e, c.plan.hdr.data(), pp, tab, ep, *sm, c.ht, 0, 1, sync);
+        }
     }
 }
 
