@@ -208,9 +208,10 @@ struct FamA {
                 contract(acc, ex, ey, ez, fx, fy, fz, R);
             }
             j += G;
-            while (j >= nk) {
-                j -= nk;
-                ++i;
+            if (j >= nk) {   // G may exceed nk when primitive slices widen the stride
+                const int di = j / nk;
+                i += di;
+                j -= di * nk;
             }
         }
     }
@@ -230,7 +231,11 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
             if (!keep_unique(q, mu, nu, la, si)) continue;
             const int64_t IJ = pair_of(mu, nu), KL = pair_of(la, si);
             if (ep.bounds[IJ] * ep.bounds[KL] < ep.threshold) continue;
-            ep.out[quartet_of(IJ, KL)] = (double)acc[e] * (fs[mu] * fs[nu] * fs[la] * fs[si]);
+            const double v = (double)acc[e] * (fs[mu] * fs[nu] * fs[la] * fs[si]);
+            if (ep.split > 1)
+                atomic_add(&ep.out[quartet_of(IJ, KL)], v);
+            else
+                ep.out[quartet_of(IJ, KL)] = v;
         }
         return;
     }
@@ -345,16 +350,19 @@ HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk)
 // dividing the wavefront).  `sub` is the lane's index inside its group; every
 // lane of the group must call this (inactive groups pass active = false) so
 // the cross-lane reduction sees the whole group.  Lane 0 of the group runs the
-// epilogue.
+// epilogue.  With nslice > 1 the group covers primitive quartets
+// slice*G + sub + k*G*nslice of the flattened bra x ket loop, and the epilogue
+// adds its partial block (J/K and the unique array are linear in it).
 template <int LA, int LB, int LC, int LD, class Real, int G, bool JK>
-HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const PairHdr* hdr, const PrimPair<Real>* pp,
-                                           const Real* tab, const EpiArgs& ep) {
+HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, int slice, int nslice, const PairHdr* hdr,
+                                           const PrimPair<Real>* pp, const Real* tab, const EpiArgs& ep) {
     using F = FamA<LA, LB, LC, LD, Real>;
     const PairHdr hb = hdr[e.x], hk = hdr[e.y];
     Real acc[F::NOUT];
 #pragma unroll
     for (int i = 0; i < F::NOUT; ++i) acc[i] = Real(0);
-    if (active) F::quartet(hb, hk, pp, tab, acc, sub, G);
+    const bool work = active && slice * G < hb.nprim * hk.nprim;
+    if (work) F::quartet(hb, hk, pp, tab, acc, slice * G + sub, G * nslice);
 #if defined(__HIP_DEVICE_COMPILE__)
     if constexpr (G > 1) {
 #pragma unroll
@@ -363,36 +371,41 @@ HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const 
             for (int i = 0; i < F::NOUT; ++i) acc[i] += __shfl_xor(acc[i], off, G);
     }
 #endif
-    if (active && sub == 0) epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, JK>(make_ctx(hb, hk), ep, acc);
+    if (work && sub == 0) epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, JK>(make_ctx(hb, hk), ep, acc);
 }
 
 template <int LA, int LB, int LC, int LD, class Real>
 HFX_HD __forceinline__ void family_a_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
                                          const EpiArgs& ep) {
     if (ep.mode == EPI_JK)
-        family_a_group<LA, LB, LC, LD, Real, 1, true>(e, true, 0, hdr, pp, tab, ep);
+        family_a_group<LA, LB, LC, LD, Real, 1, true>(e, true, 0, 0, 1, hdr, pp, tab, ep);
     else
-        family_a_group<LA, LB, LC, LD, Real, 1, false>(e, true, 0, hdr, pp, tab, ep);
+        family_a_group<LA, LB, LC, LD, Real, 1, false>(e, true, 0, 0, 1, hdr, pp, tab, ep);
 }
 
 #if !defined(__HIP_DEVICE_COMPILE__)
-// CPU emulation of a G-lane group: the G partial sums are formed one lane at a
-// time and added, exactly the partition the device uses.
+// CPU emulation of a G-lane group: the G partial sums of each primitive slice
+// are formed one lane at a time and added, exactly the partition the device
+// uses; every slice with work runs its own epilogue.
 template <int LA, int LB, int LC, int LD, class Real>
 inline void family_a_group_emu(uint2 e, int G, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
                                const EpiArgs& ep) {
     using F = FamA<LA, LB, LC, LD, Real>;
     const PairHdr hb = hdr[e.x], hk = hdr[e.y];
-    Real tot[F::NOUT] = {};
-    for (int sub = 0; sub < G; ++sub) {
-        Real acc[F::NOUT] = {};
-        F::quartet(hb, hk, pp, tab, acc, sub, G);
-        for (int i = 0; i < F::NOUT; ++i) tot[i] += acc[i];
+    const int nslice = ep.split > 1 ? ep.split : 1;
+    for (int slice = 0; slice < nslice; ++slice) {
+        if (slice * G >= hb.nprim * hk.nprim) break;
+        Real tot[F::NOUT] = {};
+        for (int sub = 0; sub < G; ++sub) {
+            Real acc[F::NOUT] = {};
+            F::quartet(hb, hk, pp, tab, acc, slice * G + sub, G * nslice);
+            for (int i = 0; i < F::NOUT; ++i) tot[i] += acc[i];
+        }
+        if (ep.mode == EPI_JK)
+            epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, true>(make_ctx(hb, hk), ep, tot);
+        else
+            epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, false>(make_ctx(hb, hk), ep, tot);
     }
-    if (ep.mode == EPI_JK)
-        epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, true>(make_ctx(hb, hk), ep, tot);
-    else
-        epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, false>(make_ctx(hb, hk), ep, tot);
 }
 #endif
 
@@ -406,7 +419,13 @@ struct FamBShape {
     static constexpr int NAB = NA * NB, NCD = NC * ND;
     static constexpr int NHAB = nherm(LAB), NR = nherm(L);
     static constexpr int KB = L <= 8 ? 4 : 1;   // ket primitive pairs processed per step
-    static constexpr int BUDGET = 5600 - (KB - 1) * 2 * NR;   // doubles for g + acc per chunk
+    // epilogue staging (doubles): per-function scales of the four shells, the
+    // bra / ket function-pair bounds, the J densities over ab and cd, and per
+    // spin the four exchange density blocks D_ac, D_ad, D_bc, D_bd
+    static constexpr int O_FS = 0, O_BAB = NA + NB + NC + ND, O_BCD = O_BAB + NAB, O_DJAB = O_BCD + NCD,
+                         O_DJCD = O_DJAB + NAB, O_DK = O_DJCD + NCD, NDK = (NA + NB) * (NC + ND),
+                         NEPI = O_DK + 2 * NDK;
+    static constexpr int BUDGET = 5600 - (KB - 1) * 2 * NR - NEPI;   // doubles for g + acc per chunk
     static constexpr int CDC0 = BUDGET / (NHAB + NAB);
     static constexpr int CDC = CDC0 < NCD ? CDC0 : NCD;
     static constexpr int NCHUNK = (NCD + CDC - 1) / CDC;
@@ -433,7 +452,58 @@ struct FamBSmem {
     unsigned char ht[S::NR][3];        // packed Hermite index -> (t, u, v)
     unsigned char abc[S::NAB][6];      // bra component pair -> (ax, ay, az, bx, by, bz)
     unsigned char cdc[S::NCD][6];      // ket component pair -> (cx, cy, cz, dx, dy, dz)
+    double epi[S::NEPI];               // epilogue staging, per quartet (offsets in FamBShape)
 };
+
+// Stages what the epilogue of one quartet reads: function scales, pair bounds
+// (not for EPI_DIAG, which is producing them) and, for J/K, the density
+// blocks, so the epilogue reads LDS instead of HBM / L2.
+template <int LA, int LB, int LC, int LD, class Real, bool JK>
+HFX_HD __forceinline__ void famb_stage_epi(FamBSmem<LA, LB, LC, LD, Real>& sm, const QuartetCtx& q,
+                                           const EpiArgs& ep, int lane, int nlanes) {
+    using S = FamBShape<LA, LB, LC, LD>;
+    double* E = sm.epi;
+    const int K = ep.K;
+    for (int w = lane; w < S::O_BAB; w += nlanes) {
+        const int fn = w < S::NA ? q.fn1 + w
+                     : w < S::NA + S::NB ? q.fn2 + (w - S::NA)
+                     : w < S::NA + S::NB + S::NC ? q.fn3 + (w - S::NA - S::NB)
+                     : q.fn4 + (w - S::NA - S::NB - S::NC);
+        E[S::O_FS + w] = ep.fscale[fn];
+    }
+    if (ep.mode == EPI_DIAG) return;
+    for (int w = lane; w < S::NAB; w += nlanes)
+        E[S::O_BAB + w] = ep.bounds[pair_of(q.fn1 + w / S::NB, q.fn2 + w % S::NB)];
+    for (int w = lane; w < S::NCD; w += nlanes)
+        E[S::O_BCD + w] = ep.bounds[pair_of(q.fn3 + w / S::ND, q.fn4 + w % S::ND)];
+    if constexpr (JK) {
+        for (int w = lane; w < S::NAB; w += nlanes)
+            E[S::O_DJAB + w] = ep.Dj[(q.fn1 + w / S::NB) * K + q.fn2 + w % S::NB];
+        for (int w = lane; w < S::NCD; w += nlanes)
+            E[S::O_DJCD + w] = ep.Dj[(q.fn3 + w / S::ND) * K + q.fn4 + w % S::ND];
+        for (int spin = 0; spin < ep.nspin; ++spin) {
+            const double* D = spin == 0 ? ep.Da : ep.Db;
+            double* dk = E + S::O_DK + spin * S::NDK;
+            // [D_ac: NA x NC][D_ad: NA x ND][D_bc: NB x NC][D_bd: NB x ND]
+            for (int w = lane; w < S::NDK; w += nlanes) {
+                int row, col;
+                if (w < S::NA * S::NC) {
+                    row = q.fn1 + w / S::NC; col = q.fn3 + w % S::NC;
+                } else if (w < S::NA * (S::NC + S::ND)) {
+                    const int x = w - S::NA * S::NC;
+                    row = q.fn1 + x / S::ND; col = q.fn4 + x % S::ND;
+                } else if (w < S::NA * (S::NC + S::ND) + S::NB * S::NC) {
+                    const int x = w - S::NA * (S::NC + S::ND);
+                    row = q.fn2 + x / S::NC; col = q.fn3 + x % S::NC;
+                } else {
+                    const int x = w - S::NA * (S::NC + S::ND) - S::NB * S::NC;
+                    row = q.fn2 + x / S::ND; col = q.fn4 + x % S::ND;
+                }
+                dk[w] = D[row * K + col];
+            }
+        }
+    }
+}
 
 // Fills the decode tables of a family-B workgroup (lanes stride the entries).
 template <int LA, int LB, int LC, int LD, class Real>
@@ -486,26 +556,35 @@ HFX_HD __forceinline__ void build_e_lanes(Real* dst, const PrimPair<Real>& pp, b
     });
 }
 
-// One wavefront (or one CPU thread with nlanes = 1) computes one shell quartet.
+// One wavefront (or one CPU thread with nlanes = 1) computes one shell quartet,
+// or, with nslice > 1, the part of it from bra primitive pairs slice,
+// slice + nslice, ... (the epilogue then adds its partial block).  chunk >= 0
+// restricts the work to that chunk of ket components (chunk < 0: all chunks),
+// so the chunks of a high-L quartet can run on different wavefronts.
 // `sync()` must be a wavefront/workgroup barrier on the device.
 template <int LA, int LB, int LC, int LD, class Real, bool JK, class Sync>
-HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab, const EpiArgs& ep,
-                         FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht, int lane, int nlanes, Sync sync) {
+HFX_HD void family_b_one(uint2 e, int slice, int nslice, int chunk, const PairHdr* hdr, const PrimPair<Real>* pp,
+                         const Real* tab, const EpiArgs& ep, FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht,
+                         int lane, int nlanes, Sync sync) {
     using S = FamBShape<LA, LB, LC, LD>;
     constexpr int SB = (LA + 1) * (LB + 1) * (S::LAB + 1);
     constexpr int SK = (LC + 1) * (LD + 1) * (S::LCD + 1);
     const PairHdr hb = hdr[e.x], hk = hdr[e.y];
+    if (slice >= hb.nprim) return;   // uniform over the workgroup: no bra primitives in this slice
     const QuartetCtx q = make_ctx(hb, hk);
     const double* fs = ep.fscale;
     const int K = ep.K;
     const double fdeg = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
     famb_tables<LA, LB, LC, LD, Real>(sm, ht, lane, nlanes);
+    famb_stage_epi<LA, LB, LC, LD, Real, JK>(sm, q, ep, lane, nlanes);
 
-    for (int ch = 0; ch < S::NCHUNK; ++ch) {
+    const int ch_begin = chunk < 0 ? 0 : chunk, ch_end = chunk < 0 ? S::NCHUNK : chunk + 1;
+    for (int ch = ch_begin; ch < ch_end; ++ch) {
         const int cd0 = ch * S::CDC;
         const int ncd = (S::NCD - cd0) < S::CDC ? (S::NCD - cd0) : S::CDC;
+        if (ch > ch_begin) sync();   // the previous chunk's epilogue has read sm.acc
         for (int w = lane; w < S::NAB * S::CDC; w += nlanes) sm.acc[w] = Real(0);
-        for (int i = 0; i < hb.nprim; ++i) {
+        for (int i = slice; i < hb.nprim; i += nslice) {
             const PrimPair<Real> b = pp[hb.prim_off + i];
             sync();
             build_e_lanes<LA, LB, Real>(sm.eb, b, false, lane, nlanes);
@@ -627,10 +706,10 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
         }
         sync();
         // ---------------- epilogue for this chunk (values read from sm.acc)
-        auto val = [&](int ab, int cdl) -> double {
-            const int a = ab / S::NB, bb = ab % S::NB, cd = cd0 + cdl, c = cd / S::ND, d = cd % S::ND;
-            const int mu = q.fn1 + a, nu = q.fn2 + bb, la = q.fn3 + c, si = q.fn4 + d;
-            return (double)sm.acc[ab * S::CDC + cdl] * (fs[mu] * fs[nu] * fs[la] * fs[si]);
+        const double* E = sm.epi;
+        auto fs4 = [&](int ab, int cd) -> double {
+            return E[S::O_FS + ab / S::NB] * E[S::O_FS + S::NA + ab % S::NB] *
+                   E[S::O_FS + S::NA + S::NB + cd / S::ND] * E[S::O_FS + S::NA + S::NB + S::NC + cd % S::ND];
         };
         if (!JK && ep.mode == EPI_UNIQUE) {
             for (int w = lane; w < S::NAB * ncd; w += nlanes) {
@@ -638,9 +717,13 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
                 const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;
                 const int la = q.fn3 + cd / S::ND, si = q.fn4 + cd % S::ND;
                 if (!keep_unique(q, mu, nu, la, si)) continue;
-                const int64_t IJ = pair_of(mu, nu), KL = pair_of(la, si);
-                if (ep.bounds[IJ] * ep.bounds[KL] < ep.threshold) continue;
-                ep.out[quartet_of(IJ, KL)] = val(ab, cdl);
+                if (E[S::O_BAB + ab] * E[S::O_BCD + cd] < ep.threshold) continue;
+                const double v = (double)sm.acc[ab * S::CDC + cdl] * fs4(ab, cd);
+                const int64_t o = quartet_of(pair_of(mu, nu), pair_of(la, si));
+                if (nslice > 1)
+                    atomic_add(&ep.out[o], v);
+                else
+                    ep.out[o] = v;
             }
         } else if (!JK && ep.mode == EPI_DIAG) {
             for (int w = lane; w < S::NAB; w += nlanes) {
@@ -648,36 +731,38 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
                 if (cd < cd0 || cd >= cd0 + ncd) continue;
                 const int mu = q.fn1 + w / S::NB, nu = q.fn2 + w % S::NB;
                 if (q.d12 && mu < nu) continue;
-                double v = val(w, cd - cd0);
+                const double v = (double)sm.acc[w * S::CDC + (cd - cd0)] * fs4(w, cd);
                 ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
             }
         } else if constexpr (JK) {
-            auto mv = [&](int ab, int cdl) -> double {
-                const int cd = cd0 + cdl;
-                const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;
-                const int la = q.fn3 + cd / S::ND, si = q.fn4 + cd % S::ND;
-                return mask_pass(ep, mu, nu, la, si) ? val(ab, cdl) : 0.0;
-            };
+            // final masked values in place, then every J/K sum reads LDS only
+            for (int w = lane; w < S::NAB * ncd; w += nlanes) {
+                const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
+                Real& a = sm.acc[ab * S::CDC + cdl];
+                a = E[S::O_BAB + ab] * E[S::O_BCD + cd] >= ep.threshold ? (Real)((double)a * fs4(ab, cd)) : Real(0);
+            }
+            sync();
+            auto v = [&](int ab, int cdl) -> double { return (double)sm.acc[ab * S::CDC + cdl]; };
             // J_ab over this chunk's cd
             for (int ab = lane; ab < S::NAB; ab += nlanes) {
                 double s = 0.0;
-                for (int cdl = 0; cdl < ncd; ++cdl) {
-                    const int cd = cd0 + cdl;
-                    s += mv(ab, cdl) * ep.Dj[(q.fn3 + cd / S::ND) * K + q.fn4 + cd % S::ND];
-                }
+                for (int cdl = 0; cdl < ncd; ++cdl) s += v(ab, cdl) * E[S::O_DJCD + cd0 + cdl];
                 atomic_add(&ep.J[(q.fn1 + ab / S::NB) * K + q.fn2 + ab % S::NB], 2.0 * fdeg * s);
             }
             for (int cdl = lane; cdl < ncd; cdl += nlanes) {
                 const int cd = cd0 + cdl;
                 double s = 0.0;
-                for (int ab = 0; ab < S::NAB; ++ab)
-                    s += mv(ab, cdl) * ep.Dj[(q.fn1 + ab / S::NB) * K + q.fn2 + ab % S::NB];
+                for (int ab = 0; ab < S::NAB; ++ab) s += v(ab, cdl) * E[S::O_DJAB + ab];
                 atomic_add(&ep.J[(q.fn3 + cd / S::ND) * K + q.fn4 + cd % S::ND], 2.0 * fdeg * s);
             }
             for (int spin = 0; spin < ep.nspin; ++spin) {
-                const double* D = spin == 0 ? ep.Da : ep.Db;
+                const double* dk = E + S::O_DK + spin * S::NDK;
+                const double* Dac = dk;
+                const double* Dad = Dac + S::NA * S::NC;
+                const double* Dbc = Dad + S::NA * S::ND;
+                const double* Dbd = Dbc + S::NB * S::NC;
                 double* Kacc = spin == 0 ? ep.Ka : ep.Kb;
-                // Kacc_ac (sum b,d) and Kacc_bc (sum a,d): targets indexed by (x, c)
+                // Kacc_ac += f sum_bd v D_bd ; Kacc_bc += f sum_ad v D_ad : targets (x, c)
                 for (int w = lane; w < (S::NA + S::NB) * S::NC; w += nlanes) {
                     const bool isA = w < S::NA * S::NC;
                     const int x = isA ? w / S::NC : (w - S::NA * S::NC) / S::NC;
@@ -690,19 +775,14 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
                         any = true;
                         const int cdl = cd - cd0;
                         if (isA) {
-                            for (int bb = 0; bb < S::NB; ++bb)
-                                s += mv(x * S::NB + bb, cdl) * D[(q.fn2 + bb) * K + q.fn4 + d];
+                            for (int bb = 0; bb < S::NB; ++bb) s += v(x * S::NB + bb, cdl) * Dbd[bb * S::ND + d];
                         } else {
-                            for (int a = 0; a < S::NA; ++a)
-                                s += mv(a * S::NB + x, cdl) * D[(q.fn1 + a) * K + q.fn4 + d];
+                            for (int a = 0; a < S::NA; ++a) s += v(a * S::NB + x, cdl) * Dad[a * S::ND + d];
                         }
                     }
-                    if (any) {
-                        const int row = isA ? q.fn1 + x : q.fn2 + x;
-                        atomic_add(&Kacc[row * K + q.fn3 + c], fdeg * s);
-                    }
+                    if (any) atomic_add(&Kacc[((isA ? q.fn1 : q.fn2) + x) * K + q.fn3 + c], fdeg * s);
                 }
-                // Kacc_ad (sum b,c) and Kacc_bd (sum a,c): targets indexed by (x, d)
+                // Kacc_ad += f sum_bc v D_bc ; Kacc_bd += f sum_ac v D_ac : targets (x, d)
                 for (int w = lane; w < (S::NA + S::NB) * S::ND; w += nlanes) {
                     const bool isA = w < S::NA * S::ND;
                     const int x = isA ? w / S::ND : (w - S::NA * S::ND) / S::ND;
@@ -715,17 +795,12 @@ HFX_HD void family_b_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, 
                         any = true;
                         const int cdl = cd - cd0;
                         if (isA) {
-                            for (int bb = 0; bb < S::NB; ++bb)
-                                s += mv(x * S::NB + bb, cdl) * D[(q.fn2 + bb) * K + q.fn3 + c];
+                            for (int bb = 0; bb < S::NB; ++bb) s += v(x * S::NB + bb, cdl) * Dbc[bb * S::NC + c];
                         } else {
-                            for (int a = 0; a < S::NA; ++a)
-                                s += mv(a * S::NB + x, cdl) * D[(q.fn1 + a) * K + q.fn3 + c];
+                            for (int a = 0; a < S::NA; ++a) s += v(a * S::NB + x, cdl) * Dac[a * S::NC + c];
                         }
                     }
-                    if (any) {
-                        const int row = isA ? q.fn1 + x : q.fn2 + x;
-                        atomic_add(&Kacc[row * K + q.fn4 + d], fdeg * s);
-                    }
+                    if (any) atomic_add(&Kacc[((isA ? q.fn1 : q.fn2) + x) * K + q.fn4 + d], fdeg * s);
                 }
             }
         }

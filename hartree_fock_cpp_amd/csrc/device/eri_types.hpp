@@ -55,6 +55,9 @@ struct EpiArgs {
     double* J;                 // J accumulator (symmetrised afterwards)
     double* Ka;                // K accumulators
     double* Kb;
+    int split;                 // primitive slices per quartet (EPI_UNIQUE / EPI_JK; 0 or 1 = none).
+                               // With more than one, each slice adds its partial block
+                               // through atomics, so EPI_UNIQUE needs a zeroed `out`.
 };
 
 __host__ __device__ inline int64_t pair_of(int a, int b) {

@@ -7,6 +7,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -96,6 +97,7 @@ struct DevLists {
     size_t off[NQCLASS][2] = {};
     int cnt[NQCLASS][2] = {};
     int bucket[NQCLASS][2][NGROUP] = {};
+    int slices[NQCLASS][2][NGROUP] = {};   // primitive slices per quartet (write / J/K launches)
     long long nprimq[NQCLASS][2] = {};   // sum over quartets of kept bra x ket primitive pairs
     bool valid = false;
     double threshold = -1.0;
@@ -178,6 +180,7 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
                 L.herm = g_consts.herm;
                 L.G = GROUP_SIZE[g];
                 L.ep = ep;
+                L.ep.split = ep.mode == EPI_DIAG ? 1 : (dl.slices[qc][p][g] > 0 ? dl.slices[qc][p][g] : 1);
                 HIPCHK(launch_eri_class(qc, p == 1, L, st));
                 off += n;
             }
@@ -189,6 +192,16 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
     return 0;
 }
 
+// Wavefront target of the primitive-slice policy: HFX_SLICE_TARGET in the
+// environment when set (0 disables slicing), else SLICE_TARGET.  Read once.
+long long slice_target() {
+    static const long long t = [] {
+        const char* e = std::getenv("HFX_SLICE_TARGET");
+        return e ? std::atoll(e) : SLICE_TARGET;
+    }();
+    return t;
+}
+
 // Uploads quartet lists into one device buffer; fills offsets / counts / buckets.
 int upload_lists(const QuartetLists& ql, DevLists& dl) {
     std::vector<uint2> all;
@@ -197,7 +210,11 @@ int upload_lists(const QuartetLists& ql, DevLists& dl) {
         for (int p = 0; p < 2; ++p) {
             dl.off[qc][p] = all.size();
             dl.cnt[qc][p] = (int)ql.list[qc][p].size();
-            for (int g = 0; g < NGROUP; ++g) dl.bucket[qc][p][g] = ql.bucket[qc][p][g];
+            for (int g = 0; g < NGROUP; ++g) {
+                dl.bucket[qc][p][g] = ql.bucket[qc][p][g];
+                dl.slices[qc][p][g] = choose_slices(qc_family_a(qc), GROUP_SIZE[g], ql.bucket[qc][p][g],
+                                                    ql.maxbra[qc][p][g], ql.maxwork[qc][p][g], slice_target());
+            }
             all.insert(all.end(), ql.list[qc][p].begin(), ql.list[qc][p].end());
         }
     if (int rc = dalloc(&dl.d, all.size())) return rc;

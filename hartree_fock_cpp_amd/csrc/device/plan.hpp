@@ -197,6 +197,8 @@ inline bool qc_family_a(int qc) {
 struct QuartetLists {
     std::vector<uint2> list[NQCLASS][2];   // [class][0 = FP64, 1 = FP32]
     int bucket[NQCLASS][2][NGROUP] = {};   // quartets per lane-group bucket (family A; family B uses bucket 3)
+    int maxbra[NQCLASS][2][NGROUP] = {};   // largest bra primitive-pair count in the bucket
+    int maxwork[NQCLASS][2][NGROUP] = {};  // largest bra x ket primitive-pair count in the bucket
     size_t total() const {
         size_t n = 0;
         for (int c = 0; c < NQCLASS; ++c) n += list[c][0].size() + list[c][1].size();
@@ -236,11 +238,40 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
                 const int wa = bp.hdr[a.x].nprim * bp.hdr[a.y].nprim, wb = bp.hdr[b.x].nprim * bp.hdr[b.y].nprim;
                 return wa > wb;
             });
-            for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = 0;
+            for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = ql.maxbra[c][p][g] = ql.maxwork[c][p][g] = 0;
             const bool fa = qc_family_a(c);
-            for (const uint2& e : v)
-                ++ql.bucket[c][p][fa ? group_bucket(bp.hdr[e.x].nprim * bp.hdr[e.y].nprim) : NGROUP - 1];
+            for (const uint2& e : v) {
+                const int w = bp.hdr[e.x].nprim * bp.hdr[e.y].nprim;
+                const int g = fa ? group_bucket(w) : NGROUP - 1;
+                ++ql.bucket[c][p][g];
+                ql.maxbra[c][p][g] = std::max(ql.maxbra[c][p][g], bp.hdr[e.x].nprim);
+                ql.maxwork[c][p][g] = std::max(ql.maxwork[c][p][g], w);
+            }
         }
+}
+
+// Primitive slices per shell quartet for one launch (one lane-group bucket of
+// n quartets).  A launch of few quartets with long primitive loops leaves most
+// of the 256 CUs idle and lasts as long as its longest quartet; slicing the
+// bra primitive loop (family B) or the flattened primitive loop (family A)
+// over S wavefronts / lane groups cuts that latency, at the price of S partial
+// epilogues.  S is chosen so the launch reaches about `target` wavefronts
+// (family B; a quarter of it for family A, whose 256-lane workgroups already
+// carry four wavefronts and whose unique-array slices pay atomics that plain
+// stores avoid; 0 disables slicing) while every slice keeps at least one bra
+// primitive (family B) or about four primitive quartets per lane (family A).
+constexpr long long SLICE_TARGET = 2048;
+constexpr int SLICE_MAX = 32;
+inline int choose_slices(bool fam_a, int G, int n, int maxbra, int maxwork, long long target = SLICE_TARGET) {
+    if (fam_a) target /= 4;
+    if (n <= 0 || target <= 0) return 1;
+    const long long waves = fam_a ? ((long long)n * G + 63) / 64 : (long long)n;
+    if (waves >= target) return 1;
+    long long S = (target + waves - 1) / waves;
+    const long long cap = fam_a ? (long long)maxwork / (4LL * G) : (long long)maxbra;
+    if (S > cap) S = cap;
+    if (S > SLICE_MAX) S = SLICE_MAX;
+    return S < 1 ? 1 : (int)S;
 }
 
 // Diagonal quartets (x|x) for the Schwarz bounds.

@@ -27,9 +27,20 @@ struct EmuCtx {
     std::vector<double> bounds, Qsh;
 };
 
+// Slices per quartet: 0 = the device's choice (choose_slices), > 0 = forced
+// (tests exercise the partial-epilogue path on small inputs).
+int g_force_slices = 0;
+
+int emu_slices(bool fam_a, int G, int n, int maxbra, int maxwork, const EpiArgs& ep) {
+    if (ep.mode == EPI_DIAG) return 1;
+    return g_force_slices > 0 ? g_force_slices : choose_slices(fam_a, G, n, maxbra, maxwork);
+}
+
 template <int QC, class Real>
-void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucket, const EpiArgs& ep) {
+void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucket, const int* maxbra,
+               const int* maxwork, const EpiArgs& ep0) {
     using C = QClass<QC>;
+    constexpr bool fam_a = use_family_a(C::LA, C::LB, C::LC, C::LD);
     const PrimPair<Real>* pp;
     const Real* tab;
     if constexpr (sizeof(Real) == 8) {
@@ -39,26 +50,37 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucke
         pp = c.plan.ppf.data();
         tab = c.boysf.data();
     }
-    if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
+    if constexpr (fam_a) {
         // replay the device's lane-group partition bucket by bucket
         size_t pos = 0;
-        for (int g = 0; g < NGROUP; ++g)
+        for (int g = 0; g < NGROUP; ++g) {
+            EpiArgs ep = ep0;
+            ep.split = emu_slices(true, GROUP_SIZE[g], bucket[g], maxbra[g], maxwork[g], ep0);
             for (int k = 0; k < bucket[g]; ++k, ++pos)
                 family_a_group_emu<C::LA, C::LB, C::LC, C::LD, Real>(list[pos], GROUP_SIZE[g], c.plan.hdr.data(), pp,
                                                                      tab, ep);
+        }
     } else {
         auto sm = std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
         auto sync = []() {};
-        for (const uint2& e : list) {
-            if (ep.mode == EPI_JK)
-                family_b_one<C::LA, C::LB, C::LC, C::LD, Real, true>(e, c.plan.hdr.data(), pp, tab, ep, *sm, c.ht, 0, 1, sync);
-            else
-                family_b_one<C::LA, C::LB, C::LC, C::LD, Real, false>(e, c.plan.hdr.data(), pp, tab, ep, *sm, c.ht, 0, 1, sync);
-        }
+        EpiArgs ep = ep0;
+        const int S = emu_slices(false, 64, (int)list.size(), maxbra[NGROUP - 1], maxwork[NGROUP - 1], ep0);
+        ep.split = S;
+        constexpr int NCH = FamBShape<C::LA, C::LB, C::LC, C::LD>::NCHUNK;
+        for (const uint2& e : list)
+            for (int sl = 0; sl < S; ++sl)
+                for (int ch = 0; ch < NCH; ++ch) {   // one device wavefront per (slice, chunk)
+                    if (ep.mode == EPI_JK)
+                        family_b_one<C::LA, C::LB, C::LC, C::LD, Real, true>(e, sl, S, ch, c.plan.hdr.data(), pp, tab,
+                                                                             ep, *sm, c.ht, 0, 1, sync);
+                    else
+                        family_b_one<C::LA, C::LB, C::LC, C::LD, Real, false>(e, sl, S, ch, c.plan.hdr.data(), pp,
+                                                                              tab, ep, *sm, c.ht, 0, 1, sync);
+                }
     }
 }
 
-using RunFn = void (*)(const EmuCtx&, const std::vector<uint2>&, const int*, const EpiArgs&);
+using RunFn = void (*)(const EmuCtx&, const std::vector<uint2>&, const int*, const int*, const int*, const EpiArgs&);
 
 template <class Real, int... QC>
 RunFn pick(std::integer_sequence<int, QC...>, int qc) {
@@ -69,9 +91,9 @@ RunFn pick(std::integer_sequence<int, QC...>, int qc) {
 void run_lists(const EmuCtx& c, const QuartetLists& ql, const EpiArgs& ep) {
     for (int qc = 0; qc < NQCLASS; ++qc) {
         if (!ql.list[qc][0].empty())
-            pick<double>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][0], ql.bucket[qc][0], ep);
+            pick<double>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][0], ql.bucket[qc][0], ql.maxbra[qc][0], ql.maxwork[qc][0], ep);
         if (!ql.list[qc][1].empty())
-            pick<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][1], ql.bucket[qc][1], ep);
+            pick<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][1], ql.bucket[qc][1], ql.maxbra[qc][1], ql.maxwork[qc][1], ep);
     }
 }
 
@@ -117,6 +139,10 @@ int make_ctx(int K, const double* xyz, const int* lmn, const int* poff, const do
 extern "C" {
 
 const char* emu_last_error() { return g_err.c_str(); }
+
+// Forces S primitive slices per quartet in the unique and J/K builds (0: the
+// device's own choice).
+void emu_set_slices(int S) { g_force_slices = S; }
 
 int emu_schwarz(EMU_BASIS_ARGS, double* bounds) {
     EmuCtx c;

@@ -146,3 +146,28 @@ def test_emu_mixed_close_to_fp64():
     K_, keep, a = _bargs(d)
     emu.emu_class_counts(*a, C.c_double(1e-12), C.c_int(1), C.c_double(1e-3), _p(n64), _p(n32))
     assert n32.sum() > 0 and n64.sum() > 0
+
+
+@pytest.mark.parametrize("S", [1, 3, 7])
+def test_emu_primitive_slices_are_exact(S):
+    """Slicing a quartet's primitive loop and adding the partial blocks (unique
+    array and J/K) reproduces the unsliced result to rounding."""
+    d = golden("h2o_ccpvdz.npz")
+    K = int(d["K"])
+    thr = float(d["threshold"])
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((K, K))
+    P = A @ A.T / K
+    emu.emu_set_slices(C.c_int(1))
+    try:
+        u1 = emu_unique(d, thr)
+        J1, K1 = emu_fock(d, thr, P)
+        emu.emu_set_slices(C.c_int(S))
+        uS = emu_unique(d, thr)
+        JS, KS = emu_fock(d, thr, P)
+    finally:
+        emu.emu_set_slices(C.c_int(0))
+    assert np.abs(uS - u1).max() < 1e-14
+    assert np.abs(JS - J1).max() < 1e-12 and np.abs(KS - K1).max() < 1e-12
+    exact = orc.eri_unique(d, thr, exact=True)
+    assert np.abs(uS - exact).max() < 1e-12
