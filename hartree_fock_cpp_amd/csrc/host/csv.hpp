@@ -81,10 +81,21 @@ inline void write_csv(const std::string& path, const std::vector<std::string>& n
                 [&](const auto& v) {
                     using T = typename std::decay_t<decltype(v)>::value_type;
                     if (r >= v.size()) return;
-                    if constexpr (std::is_same_v<T, double>)
+                    if constexpr (std::is_same_v<T, double>) {
                         f << fmt_double(v[r]);
-                    else
+                    } else if constexpr (std::is_same_v<T, std::string>) {
+                        // quoted when it holds a separator, space or line break (rapidcsv's auto-quote)
+                        const std::string& c = v[r];
+                        if (c.find_first_of(", \n") != std::string::npos) {
+                            f << '"';
+                            for (char ch : c) f << (ch == '"' ? std::string("\"\"") : std::string(1, ch));
+                            f << '"';
+                        } else {
+                            f << c;
+                        }
+                    } else {
                         f << v[r];
+                    }
                 },
                 columns[i]);
         }
