@@ -15,7 +15,16 @@ baseline times the reference's own basis_repulsion (oracle/_ref, compiled
 from the reference sources) on one host core over a bounded random sample of
 the same quartets.
 
-Usage: python bench.py [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu]
+Beside the headline line's own fields, the same JSON line carries
+  scf         SCF wall times: benzene/cc-pVDZ stored FP64; n-octane/cc-pVDZ and
+              (H2O)4/cc-pVTZ direct, FP64 vs mixed with dE, max|dF|, max|dD|
+  readme_scf  RHF/cc-pVTZ SCF wall time of the eight molecules of the
+              reference's published timing table, beside the published seconds
+  serving     the scan-serving stream at 0.7 x measured saturation: p50 / p99
+              latency, median solve time and solves/s
+(--no-extras skips them).
+
+Usage: python bench.py [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu] [--no-extras]
 """
 import argparse
 import ctypes as C
@@ -85,6 +94,110 @@ def cpu_baseline(arrs, bounds, thr, budget_s, seed=0):
                       f"workload, {t:.1f} s of single-core basis_repulsion (include/integrals.hpp:701-729)"}
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def polar_config(atoms):
+    """[(symbol, (x, y, z))] from the reference's JSON atom records (include/run.hpp:98-123)."""
+    import math
+    out = []
+    for a in atoms:
+        r = a["position"]
+        if a.get("coordinate_type", "cartesian") == "polar":
+            th, ph = r[1] * math.pi / 180.0, r[2] * math.pi / 180.0
+            out.append((a["symbol"], (r[0] * math.sin(th) * math.cos(ph), r[0] * math.sin(th) * math.sin(ph),
+                                      r[0] * math.cos(th))))
+        else:
+            out.append((a["symbol"], tuple(float(x) for x in r)))
+    return out
+
+
+def direct_scf_pair(name, atoms, basis_set, thr=1e-12, split=1e-5):
+    """FP64 vs mixed direct SCF: wall times, dE (before / after the FP64
+    polish), max|dF| of one Fock build at the FP64 density, max|dD|."""
+    import numpy as np
+    import hartree_fock_cpp_amd as hfx
+    from hartree_fock_cpp_amd.basis import basis_construction
+    from hartree_fock_cpp_amd.scf import rhf_diis
+    t = time.perf_counter()
+    r64 = rhf_diis(atoms, basis_set, eri_mode="direct", precision="fp64", screening_threshold=thr)
+    w64 = time.perf_counter() - t
+    t = time.perf_counter()
+    rmx = rhf_diis(atoms, basis_set, eri_mode="direct", precision="mixed", screening_threshold=thr,
+                   mixed_split=split, polish=True)
+    wmx = time.perf_counter() - t
+    b = hfx.Basis.from_arrays(basis_construction(atoms, basis_set))
+    P = r64["density"]
+    G = {}
+    for prec in ("fp64", "mixed"):
+        fb = hfx.FockBuilder(b, mode="direct", precision=prec, threshold=thr, mixed_split=split)
+        J, K = fb.build(P)
+        G[prec] = J - 0.5 * K
+        fb.close()
+    b.close()
+    out = {"workload": f"{name} RHF direct SCF, Schwarz {thr:g}, FP32/FP64 split {split:g}", "K": r64["K"],
+           "fp64_wall_s": w64, "mixed_wall_s": wmx, "mixed_over_fp64_speedup": w64 / wmx,
+           "fp64_iterations": r64["iterations"], "mixed_iterations": rmx["iterations"],
+           "mixed_polish_iterations": rmx["polish_iterations"],
+           "fp64_fock_s": r64["t_fock"], "mixed_fock_s": rmx["t_fock"],
+           "E_fp64": r64["E_tot"], "dE_before_polish": rmx["E_before_polish"] - r64["E_tot"],
+           "dE": rmx["E_tot"] - r64["E_tot"], "max_dF": float(np.abs(G["mixed"] - G["fp64"]).max()),
+           "max_dD": float(np.abs(rmx["density"] - P).max()), "converged": bool(r64["converged"] and rmx["converged"])}
+    log(f"  {name}: fp64 {w64:.2f} s, mixed {wmx:.2f} s, dE {out['dE']:.2e}")
+    return out
+
+
+def stored_scf(name, atoms, basis_set, thr=1e-8):
+    from hartree_fock_cpp_amd.scf import rhf_diis
+    t = time.perf_counter()
+    r = rhf_diis(atoms, basis_set, eri_mode="auto", precision="fp64", screening_threshold=thr)
+    w = time.perf_counter() - t
+    log(f"  {name}: {w:.3f} s, E {r['E_tot']:.10f}, {r['iterations']} iterations")
+    return {"wall_s": w, "E_tot": r["E_tot"], "iterations": r["iterations"], "K": r["K"],
+            "eri_s": r["t_eri"], "fock_s": r["t_fock"], "diag_s": r["t_diag"], "mode": r["mode"],
+            "converged": bool(r["converged"])}
+
+
+def readme_scf():
+    """The reference's published table: RHF/cc-pVTZ SCF wall time per molecule."""
+    with open(os.path.join(ROOT, "tests", "golden", "readme_molecules.json")) as f:
+        mols = json.load(f)
+    out = {}
+    for name, m in mols.items():
+        r = stored_scf(name, polar_config(m["atoms"]), m["basis"])
+        r.update(published_gpu_s=m["published_gpu_s"], published_cpu_s=m["published_cpu_s"],
+                 speedup_vs_published_gpu=m["published_gpu_s"] / r["wall_s"])
+        out[name] = r
+    return out
+
+
+def serving(n_cal=200, n_jobs=500, load=0.7, seed=2026):
+    """Scan-serving stream: saturation from a burst of n_cal jobs, then n_jobs
+    Poisson arrivals at load x saturation."""
+    from hartree_fock_cpp_amd.serve import Server, run_stream, scan_stream
+    kw = dict(max_iterations=300)
+    burst = scan_stream(n_cal, rate=1e6, seed=seed + 1)
+    with Server(max_inflight=16, hbm_budget=32 << 30) as s:
+        _, cal = run_stream(s, burst, **kw)
+    sat = cal["solves_per_s"]
+    rate = load * sat
+    log(f"  serving: saturation {sat:.1f} solves/s, offered {rate:.1f}/s")
+    jobs = scan_stream(n_jobs, rate=rate, seed=seed)
+    with Server(max_inflight=16, hbm_budget=32 << 30) as s:
+        res, summ = run_stream(s, jobs, **kw)
+    import numpy as np
+    svc = np.array([r["t_done"] - r["t_admit"] for r in res])
+    summ.update(saturation_solves_per_s=sat, offered_rate=rate, load=load, seed=seed,
+                median_solve_s=float(np.median(svc)),
+                p99_over_median_solve=summ["p99_latency_s"] / float(np.median(svc)),
+                fp64_jobs=sum(j["precision"] == "fp64" for j in jobs),
+                mixed_jobs=sum(j["precision"] == "mixed" for j in jobs))
+    log(f"  serving: p50 {summ['p50_latency_s']*1e3:.1f} ms, p99 {summ['p99_latency_s']*1e3:.1f} ms, "
+        f"{summ['solves_per_s']:.1f} solves/s")
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,7 +205,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the SCF wall-time, mixed-vs-FP64 and serving sections")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
+                         "for scaling HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU; multi-GPU sharding is described "
               "in DESIGN.md and not launched.", file=sys.stderr)
@@ -189,6 +317,19 @@ def main():
     }
     if cpu:
         line["gpu_over_cpu"] = line["value"] / cpu["value"]
+    if not args.no_extras:
+        log("extras: SCF wall times")
+        scf = {"benzene_ccpvdz_stored_fp64": stored_scf("benzene/cc-pVDZ", atoms, "cc-pVDZ")}
+        if cpu:
+            scf["benzene_ccpvdz_stored_fp64"]["cpu_eri_phase_estimate_s"] = q_screened / cpu["value"]
+        log("extras: mixed vs FP64 direct SCF")
+        scf["octane_ccpvdz_direct"] = direct_scf_pair("n-octane/cc-pVDZ", W.n_octane(), "cc-pVDZ")
+        scf["water4_ccpvtz_direct"] = direct_scf_pair("(H2O)4/cc-pVTZ", W.water_tetramer(), "cc-pVTZ")
+        line["scf"] = scf
+        log("extras: published-table molecules")
+        line["readme_scf"] = readme_scf()
+        log("extras: serving stream")
+        line["serving"] = serving()
     print(json.dumps(line), flush=True)
 
 

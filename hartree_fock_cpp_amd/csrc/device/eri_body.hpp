@@ -219,7 +219,7 @@ struct FamA {
 
 // Family-A epilogue over a register block acc[NOUT] (raw, unscaled).
 template <int NA, int NB, int NC, int ND, class Real, bool JK>
-HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep, const Real* acc) {
+HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep, const Real* acc, bool partial) {
     constexpr int NCD = NC * ND;
     const double* fs = ep.fscale;
     if constexpr (!JK) {
@@ -232,7 +232,7 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
             const int64_t IJ = pair_of(mu, nu), KL = pair_of(la, si);
             if (ep.bounds[IJ] * ep.bounds[KL] < ep.threshold) continue;
             const double v = (double)acc[e] * (fs[mu] * fs[nu] * fs[la] * fs[si]);
-            if (ep.split > 1)
+            if (partial)
                 atomic_add(&ep.out[quartet_of(IJ, KL)], v);
             else
                 ep.out[quartet_of(IJ, KL)] = v;
@@ -346,18 +346,19 @@ HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk)
     return q;
 }
 
-// One shell quartet per group of G lanes (G = 1, 4, 16 or 64, a power of two
+// One work unit per group of G lanes (G = 1, 4, 16 or 64, a power of two
 // dividing the wavefront).  `sub` is the lane's index inside its group; every
 // lane of the group must call this (inactive groups pass active = false) so
 // the cross-lane reduction sees the whole group.  Lane 0 of the group runs the
-// epilogue.  With nslice > 1 the group covers primitive quartets
-// slice*G + sub + k*G*nslice of the flattened bra x ket loop, and the epilogue
-// adds its partial block (J/K and the unique array are linear in it).
+// epilogue.  A unit that is slice s of S covers primitive quartets
+// s*G + sub + k*G*S of the flattened bra x ket loop, and its epilogue adds a
+// partial block (J/K and the unique array are linear in it).
 template <int LA, int LB, int LC, int LD, class Real, int G, bool JK>
-HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, int slice, int nslice, const PairHdr* hdr,
-                                           const PrimPair<Real>* pp, const Real* tab, const EpiArgs& ep) {
+HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const PairHdr* hdr, const PrimPair<Real>* pp,
+                                           const Real* tab, const EpiArgs& ep) {
     using F = FamA<LA, LB, LC, LD, Real>;
-    const PairHdr hb = hdr[e.x], hk = hdr[e.y];
+    const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
+    const int slice = unit_slice(e), nslice = unit_nslice(e);
     Real acc[F::NOUT];
 #pragma unroll
     for (int i = 0; i < F::NOUT; ++i) acc[i] = Real(0);
@@ -371,41 +372,39 @@ HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, int sl
             for (int i = 0; i < F::NOUT; ++i) acc[i] += __shfl_xor(acc[i], off, G);
     }
 #endif
-    if (work && sub == 0) epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, JK>(make_ctx(hb, hk), ep, acc);
+    if (work && sub == 0)
+        epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, JK>(make_ctx(hb, hk), ep, acc, nslice > 1);
 }
 
 template <int LA, int LB, int LC, int LD, class Real>
 HFX_HD __forceinline__ void family_a_one(uint2 e, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
                                          const EpiArgs& ep) {
     if (ep.mode == EPI_JK)
-        family_a_group<LA, LB, LC, LD, Real, 1, true>(e, true, 0, 0, 1, hdr, pp, tab, ep);
+        family_a_group<LA, LB, LC, LD, Real, 1, true>(e, true, 0, hdr, pp, tab, ep);
     else
-        family_a_group<LA, LB, LC, LD, Real, 1, false>(e, true, 0, 0, 1, hdr, pp, tab, ep);
+        family_a_group<LA, LB, LC, LD, Real, 1, false>(e, true, 0, hdr, pp, tab, ep);
 }
 
 #if !defined(__HIP_DEVICE_COMPILE__)
-// CPU emulation of a G-lane group: the G partial sums of each primitive slice
-// are formed one lane at a time and added, exactly the partition the device
-// uses; every slice with work runs its own epilogue.
+// CPU emulation of a G-lane group: the G partial sums of the unit are formed
+// one lane at a time and added, exactly the partition the device uses.
 template <int LA, int LB, int LC, int LD, class Real>
 inline void family_a_group_emu(uint2 e, int G, const PairHdr* hdr, const PrimPair<Real>* pp, const Real* tab,
                                const EpiArgs& ep) {
     using F = FamA<LA, LB, LC, LD, Real>;
-    const PairHdr hb = hdr[e.x], hk = hdr[e.y];
-    const int nslice = ep.split > 1 ? ep.split : 1;
-    for (int slice = 0; slice < nslice; ++slice) {
-        if (slice * G >= hb.nprim * hk.nprim) break;
-        Real tot[F::NOUT] = {};
-        for (int sub = 0; sub < G; ++sub) {
-            Real acc[F::NOUT] = {};
-            F::quartet(hb, hk, pp, tab, acc, slice * G + sub, G * nslice);
-            for (int i = 0; i < F::NOUT; ++i) tot[i] += acc[i];
-        }
-        if (ep.mode == EPI_JK)
-            epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, true>(make_ctx(hb, hk), ep, tot);
-        else
-            epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, false>(make_ctx(hb, hk), ep, tot);
+    const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
+    const int slice = unit_slice(e), nslice = unit_nslice(e);
+    if (slice * G >= hb.nprim * hk.nprim) return;
+    Real tot[F::NOUT] = {};
+    for (int sub = 0; sub < G; ++sub) {
+        Real acc[F::NOUT] = {};
+        F::quartet(hb, hk, pp, tab, acc, slice * G + sub, G * nslice);
+        for (int i = 0; i < F::NOUT; ++i) tot[i] += acc[i];
     }
+    if (ep.mode == EPI_JK)
+        epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, true>(make_ctx(hb, hk), ep, tot, nslice > 1);
+    else
+        epilogue_regs<F::NA, F::NB, F::NC, F::ND, Real, false>(make_ctx(hb, hk), ep, tot, nslice > 1);
 }
 #endif
 
@@ -556,20 +555,21 @@ HFX_HD __forceinline__ void build_e_lanes(Real* dst, const PrimPair<Real>& pp, b
     });
 }
 
-// One wavefront (or one CPU thread with nlanes = 1) computes one shell quartet,
-// or, with nslice > 1, the part of it from bra primitive pairs slice,
-// slice + nslice, ... (the epilogue then adds its partial block).  chunk >= 0
+// One wavefront (or one CPU thread with nlanes = 1) computes one work unit: a
+// shell quartet or, for slice s of S, the part of it from bra primitive pairs
+// s, s + S, ... (the epilogue then adds its partial block).  chunk >= 0
 // restricts the work to that chunk of ket components (chunk < 0: all chunks),
 // so the chunks of a high-L quartet can run on different wavefronts.
 // `sync()` must be a wavefront/workgroup barrier on the device.
 template <int LA, int LB, int LC, int LD, class Real, bool JK, class Sync>
-HFX_HD void family_b_one(uint2 e, int slice, int nslice, int chunk, const PairHdr* hdr, const PrimPair<Real>* pp,
+HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<Real>* pp,
                          const Real* tab, const EpiArgs& ep, FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht,
                          int lane, int nlanes, Sync sync) {
     using S = FamBShape<LA, LB, LC, LD>;
     constexpr int SB = (LA + 1) * (LB + 1) * (S::LAB + 1);
     constexpr int SK = (LC + 1) * (LD + 1) * (S::LCD + 1);
-    const PairHdr hb = hdr[e.x], hk = hdr[e.y];
+    const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
+    const int slice = unit_slice(e), nslice = unit_nslice(e);
     if (slice >= hb.nprim) return;   // uniform over the workgroup: no bra primitives in this slice
     const QuartetCtx q = make_ctx(hb, hk);
     const double* fs = ep.fscale;

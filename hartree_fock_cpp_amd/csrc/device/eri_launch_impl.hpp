@@ -15,23 +15,19 @@ namespace hfx {
 
 constexpr int FAM_A_BLOCK = 256;
 
-// G lanes per shell quartet; FAM_A_BLOCK / G groups per workgroup.  Group u
-// takes quartet u / S, primitive slice u % S (S = ep.split, 1 without slices).
+// G lanes per work unit; FAM_A_BLOCK / G units per workgroup.
 template <int LA, int LB, int LC, int LD, class Real, int G, bool JK>
 __global__ __launch_bounds__(FAM_A_BLOCK) void k_fam_a(const uint2* __restrict__ list, int n,
                                                         const PairHdr* __restrict__ hdr,
                                                         const PrimPair<Real>* __restrict__ pp,
                                                         const Real* __restrict__ tab, EpiArgs ep) {
-    const int S = ep.split > 1 ? ep.split : 1;
-    const long long u = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const int q = (int)(u / S), slice = (int)(u % S);
-    const bool active = q < n;
-    family_a_group<LA, LB, LC, LD, Real, G, JK>(list[active ? q : n - 1], active, (int)(threadIdx.x % G), slice, S,
-                                                hdr, pp, tab, ep);
+    const int u = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool active = u < n;
+    family_a_group<LA, LB, LC, LD, Real, G, JK>(list[active ? u : n - 1], active, (int)(threadIdx.x % G), hdr, pp,
+                                                tab, ep);
 }
 
-// One wavefront per (shell quartet, primitive slice, ket-component chunk),
-// grid-strided.
+// One wavefront per (work unit, ket-component chunk), grid-strided.
 template <int LA, int LB, int LC, int LD, class Real, bool JK>
 __global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
@@ -39,14 +35,10 @@ __global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, in
     __shared__ FamBSmem<LA, LB, LC, LD, Real> sm;
     auto sync = []() { __syncthreads(); };
     constexpr int NCH = FamBShape<LA, LB, LC, LD>::NCHUNK;
-    const int S = ep.split > 1 ? ep.split : 1;
-    const long long per_q = (long long)S * NCH;
-    const long long total = (long long)n * per_q;
-    for (long long u = blockIdx.x; u < total; u += gridDim.x) {
-        const long long r = u % per_q;
-        family_b_one<LA, LB, LC, LD, Real, JK>(list[u / per_q], (int)(r / NCH), S, (int)(r % NCH), hdr, pp, tab, ep,
-                                               sm, *ht, (int)threadIdx.x, 64, sync);
-    }
+    const long long total = (long long)n * NCH;
+    for (long long u = blockIdx.x; u < total; u += gridDim.x)
+        family_b_one<LA, LB, LC, LD, Real, JK>(list[u / NCH], (int)(u % NCH), hdr, pp, tab, ep, sm, *ht,
+                                               (int)threadIdx.x, 64, sync);
 }
 
 template <int QC, class Real, bool JK>
@@ -55,7 +47,7 @@ hipError_t launch_class_mode(const ClassLaunch& L, hipStream_t st) {
     const PrimPair<Real>* pp = static_cast<const PrimPair<Real>*>(L.prim);
     const Real* tab = static_cast<const Real*>(L.boys);
     if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
-        const long long lanes = (long long)L.n * (L.ep.split > 1 ? L.ep.split : 1) * L.G;
+        const long long lanes = (long long)L.n * L.G;
         const int grid = (int)((lanes + FAM_A_BLOCK - 1) / FAM_A_BLOCK);
         switch (L.G) {
             case 1: k_fam_a<C::LA, C::LB, C::LC, C::LD, Real, 1, JK><<<grid, FAM_A_BLOCK, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.ep); break;
@@ -65,8 +57,7 @@ hipError_t launch_class_mode(const ClassLaunch& L, hipStream_t st) {
             default: return hipErrorInvalidValue;
         }
     } else {
-        const long long units = (long long)L.n * (L.ep.split > 1 ? L.ep.split : 1) *
-                                FamBShape<C::LA, C::LB, C::LC, C::LD>::NCHUNK;
+        const long long units = (long long)L.n * FamBShape<C::LA, C::LB, C::LC, C::LD>::NCHUNK;
         const int grid = units < 65536 ? (int)units : 65536;
         k_fam_b<C::LA, C::LB, C::LC, C::LD, Real, JK><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.herm, L.ep);
     }

@@ -55,10 +55,18 @@ struct EpiArgs {
     double* J;                 // J accumulator (symmetrised afterwards)
     double* Ka;                // K accumulators
     double* Kb;
-    int split;                 // primitive slices per quartet (EPI_UNIQUE / EPI_JK; 0 or 1 = none).
-                               // With more than one, each slice adds its partial block
-                               // through atomics, so EPI_UNIQUE needs a zeroed `out`.
 };
+
+// A work unit of a launch list: a shell quartet (bra, ket pair ids < 2^24)
+// or one primitive slice of it, with the slice index in the top 8 bits of .x
+// and the slice count in the top 8 bits of .y (0 reads as 1: a whole
+// quartet).  Slices add partial blocks in the epilogue through atomics, so
+// EPI_UNIQUE then needs a zeroed output.
+constexpr unsigned UNIT_ID_MASK = 0xFFFFFFu;
+__host__ __device__ inline unsigned unit_bra(uint2 e) { return e.x & UNIT_ID_MASK; }
+__host__ __device__ inline unsigned unit_ket(uint2 e) { return e.y & UNIT_ID_MASK; }
+__host__ __device__ inline int unit_slice(uint2 e) { return (int)(e.x >> 24); }
+__host__ __device__ inline int unit_nslice(uint2 e) { return (e.y >> 24) ? (int)(e.y >> 24) : 1; }
 
 __host__ __device__ inline int64_t pair_of(int a, int b) {
     return a >= b ? (int64_t)a * (a + 1) / 2 + b : (int64_t)b * (b + 1) / 2 + a;

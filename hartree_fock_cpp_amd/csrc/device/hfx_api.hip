@@ -95,9 +95,9 @@ void set_error(const std::string& msg) { g_err = msg; }
 struct DevLists {
     uint2* d = nullptr;
     size_t off[NQCLASS][2] = {};
-    int cnt[NQCLASS][2] = {};
+    int cnt[NQCLASS][2] = {};            // work units per list (a quartet or a primitive slice of one)
+    long long nquart[NQCLASS][2] = {};   // shell quartets per list
     int bucket[NQCLASS][2][NGROUP] = {};
-    int slices[NQCLASS][2][NGROUP] = {};   // primitive slices per quartet (write / J/K launches)
     long long nprimq[NQCLASS][2] = {};   // sum over quartets of kept bra x ket primitive pairs
     bool valid = false;
     double threshold = -1.0;
@@ -180,7 +180,6 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
                 L.herm = g_consts.herm;
                 L.G = GROUP_SIZE[g];
                 L.ep = ep;
-                L.ep.split = ep.mode == EPI_DIAG ? 1 : (dl.slices[qc][p][g] > 0 ? dl.slices[qc][p][g] : 1);
                 HIPCHK(launch_eri_class(qc, p == 1, L, st));
                 off += n;
             }
@@ -192,8 +191,11 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
     return 0;
 }
 
-// Wavefront target of the primitive-slice policy: HFX_SLICE_TARGET in the
-// environment when set (0 disables slicing), else SLICE_TARGET.  Read once.
+// Wavefront target of the primitive-slice policy for J/K (direct Fock build)
+// plans: HFX_SLICE_TARGET in the environment when set (0 disables slicing),
+// else SLICE_TARGET.  Read once.  Unique-array plans are not sliced: their
+// slices would add through atomics into the output, and on the measured
+// builds (benzene/cc-pVDZ) whole quartets with plain stores were faster.
 long long slice_target() {
     static const long long t = [] {
         const char* e = std::getenv("HFX_SLICE_TARGET");
@@ -202,20 +204,26 @@ long long slice_target() {
     return t;
 }
 
-// Uploads quartet lists into one device buffer; fills offsets / counts / buckets.
-int upload_lists(const QuartetLists& ql, DevLists& dl) {
+// Uploads the work units of the quartet lists (expand_units with the given
+// wavefront target; 0 for the Schwarz diagonal, whose epilogue needs whole
+// quartets) into one device buffer; fills offsets / counts / buckets in units.
+int upload_lists(const hfx_basis* b, const QuartetLists& ql, long long target, DevLists& dl) {
     std::vector<uint2> all;
     all.reserve(ql.total());
     for (int qc = 0; qc < NQCLASS; ++qc)
         for (int p = 0; p < 2; ++p) {
             dl.off[qc][p] = all.size();
-            dl.cnt[qc][p] = (int)ql.list[qc][p].size();
+            const std::vector<uint2>& v = ql.list[qc][p];
+            size_t pos = 0;
             for (int g = 0; g < NGROUP; ++g) {
-                dl.bucket[qc][p][g] = ql.bucket[qc][p][g];
-                dl.slices[qc][p][g] = choose_slices(qc_family_a(qc), GROUP_SIZE[g], ql.bucket[qc][p][g],
-                                                    ql.maxbra[qc][p][g], ql.maxwork[qc][p][g], slice_target());
+                const size_t before = all.size();
+                expand_units(b->plan, qc_family_a(qc), GROUP_SIZE[g], v.data() + pos, ql.bucket[qc][p][g], target,
+                             all);
+                pos += ql.bucket[qc][p][g];
+                dl.bucket[qc][p][g] = (int)(all.size() - before);
             }
-            all.insert(all.end(), ql.list[qc][p].begin(), ql.list[qc][p].end());
+            dl.cnt[qc][p] = (int)(all.size() - dl.off[qc][p]);
+            dl.nquart[qc][p] = (long long)v.size();
         }
     if (int rc = dalloc(&dl.d, all.size())) return rc;
     if (!all.empty()) HIPCHK(hipMemcpy(dl.d, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
@@ -224,12 +232,12 @@ int upload_lists(const QuartetLists& ql, DevLists& dl) {
 
 // Plans the screened quartets of a basis and uploads them (cached per basis
 // for the FP64 unique build).
-int make_plan(hfx_basis* b, double thr, int mixed, double split, DevLists& dl) {
+int make_plan(hfx_basis* b, double thr, int mixed, double split, long long target, DevLists& dl) {
     dfree(dl.d);
     dl.valid = false;
     QuartetLists ql;
     plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql);
-    if (int rc = upload_lists(ql, dl)) return rc;
+    if (int rc = upload_lists(b, ql, target, dl)) return rc;
     for (int qc = 0; qc < NQCLASS; ++qc)
         for (int p = 0; p < 2; ++p) {
             long long s = 0;
@@ -260,7 +268,7 @@ int compute_bounds(hfx_basis* b) {
         });
     }
     DevLists dl;
-    if (int rc = upload_lists(ql, dl)) return rc;
+    if (int rc = upload_lists(b, ql, 0, dl)) return rc;
     HIPCHK(launch_zero(b->d_bounds, P, b->stream));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_DIAG;
@@ -283,7 +291,7 @@ int eri_unique_into(hfx_basis* b, double threshold, double* d_out, hipStream_t s
     if (int rc = compute_bounds(b)) return rc;
     DevLists& dl = b->unique_plan;
     if (!dl.valid || dl.threshold != threshold)
-        if (int rc = make_plan(b, threshold, 0, 0.0, dl)) return rc;
+        if (int rc = make_plan(b, threshold, 0, 0.0, 0, dl)) return rc;
     HIPCHK(launch_zero(d_out, n_unique(b->plan.K), st));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_UNIQUE;
@@ -463,12 +471,12 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     if (opts->precision == HFX_PREC_MIXED) mode = HFX_MODE_DIRECT;   // the mixed form is a direct build
     {
         const int mixed = opts->precision == HFX_PREC_MIXED;
-        if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split, f->plan)) {
+        if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split, slice_target(), f->plan)) {
             hfx_fock_destroy(f);
             return rc;
         }
         size_t n = 0;
-        for (int qc = 0; qc < NQCLASS; ++qc) n += f->plan.cnt[qc][0] + f->plan.cnt[qc][1];
+        for (int qc = 0; qc < NQCLASS; ++qc) n += f->plan.nquart[qc][0] + f->plan.nquart[qc][1];
         f->nq = n;
     }
     const size_t stored_bytes = Q * sizeof(double) + mats;
@@ -523,8 +531,8 @@ int hfx_fock_info(const hfx_fock* f, int* mode, size_t* device_bytes, size_t* n_
 int hfx_fock_class_counts(const hfx_fock* f, long long* n64, long long* n32) {
     if (!f) return fail(HFX_ERR_INVALID, "null fock");
     for (int qc = 0; qc < NQCLASS; ++qc) {
-        if (n64) n64[qc] = f->plan.cnt[qc][0];
-        if (n32) n32[qc] = f->plan.cnt[qc][1];
+        if (n64) n64[qc] = f->plan.nquart[qc][0];
+        if (n32) n32[qc] = f->plan.nquart[qc][1];
     }
     return 0;
 }

@@ -197,8 +197,6 @@ inline bool qc_family_a(int qc) {
 struct QuartetLists {
     std::vector<uint2> list[NQCLASS][2];   // [class][0 = FP64, 1 = FP32]
     int bucket[NQCLASS][2][NGROUP] = {};   // quartets per lane-group bucket (family A; family B uses bucket 3)
-    int maxbra[NQCLASS][2][NGROUP] = {};   // largest bra primitive-pair count in the bucket
-    int maxwork[NQCLASS][2][NGROUP] = {};  // largest bra x ket primitive-pair count in the bucket
     size_t total() const {
         size_t n = 0;
         for (int c = 0; c < NQCLASS; ++c) n += list[c][0].size() + list[c][1].size();
@@ -238,40 +236,54 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
                 const int wa = bp.hdr[a.x].nprim * bp.hdr[a.y].nprim, wb = bp.hdr[b.x].nprim * bp.hdr[b.y].nprim;
                 return wa > wb;
             });
-            for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = ql.maxbra[c][p][g] = ql.maxwork[c][p][g] = 0;
+            for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = 0;
             const bool fa = qc_family_a(c);
-            for (const uint2& e : v) {
-                const int w = bp.hdr[e.x].nprim * bp.hdr[e.y].nprim;
-                const int g = fa ? group_bucket(w) : NGROUP - 1;
-                ++ql.bucket[c][p][g];
-                ql.maxbra[c][p][g] = std::max(ql.maxbra[c][p][g], bp.hdr[e.x].nprim);
-                ql.maxwork[c][p][g] = std::max(ql.maxwork[c][p][g], w);
-            }
+            for (const uint2& e : v) ++ql.bucket[c][p][fa ? group_bucket(bp.hdr[e.x].nprim * bp.hdr[e.y].nprim) : NGROUP - 1];
         }
 }
 
-// Primitive slices per shell quartet for one launch (one lane-group bucket of
-// n quartets).  A launch of few quartets with long primitive loops leaves most
-// of the 256 CUs idle and lasts as long as its longest quartet; slicing the
-// bra primitive loop (family B) or the flattened primitive loop (family A)
-// over S wavefronts / lane groups cuts that latency, at the price of S partial
-// epilogues.  S is chosen so the launch reaches about `target` wavefronts
-// (family B; a quarter of it for family A, whose 256-lane workgroups already
-// carry four wavefronts and whose unique-array slices pay atomics that plain
-// stores avoid; 0 disables slicing) while every slice keeps at least one bra
-// primitive (family B) or about four primitive quartets per lane (family A).
+// Work units of one launch (one lane-group bucket of n quartets, sorted by
+// primitive work w = bra x ket kept primitive pairs).  A launch lasts at
+// least as long as its longest unit: a few very contracted quartets, or a
+// launch of few quartets, leave most of the 256 CUs idle.  With R units
+// resident on the device at once (`target` wavefronts: one unit per wavefront
+// in family B, 64 / G units per wavefront in family A) the launch needs about
+// total work / R per slot, so a quartet is cut into nslice = ceil(w / W)
+// primitive slices -- of the bra primitive loop (family B) or of the
+// flattened primitive loop (family A) -- with
+//   W = max(total work / R, w_min),
+// w_min = one bra primitive (family B) or four primitive quartets per lane
+// (family A).  Launches that fill the device keep their quartets whole
+// except for outliers; slices add partial blocks in the epilogue (atomics).
+// target <= 0 disables slicing.
+//
+// A unit is the quartet's uint2 list entry with the slice index in the top
+// 8 bits of .x and the slice count in the top 8 bits of .y (pair ids < 2^24;
+// a count of 0 reads as 1, so plain entries are whole quartets).
 constexpr long long SLICE_TARGET = 2048;
-constexpr int SLICE_MAX = 32;
-inline int choose_slices(bool fam_a, int G, int n, int maxbra, int maxwork, long long target = SLICE_TARGET) {
-    if (fam_a) target /= 4;
-    if (n <= 0 || target <= 0) return 1;
-    const long long waves = fam_a ? ((long long)n * G + 63) / 64 : (long long)n;
-    if (waves >= target) return 1;
-    long long S = (target + waves - 1) / waves;
-    const long long cap = fam_a ? (long long)maxwork / (4LL * G) : (long long)maxbra;
-    if (S > cap) S = cap;
-    if (S > SLICE_MAX) S = SLICE_MAX;
-    return S < 1 ? 1 : (int)S;
+constexpr int SLICE_MAX = 255;
+
+inline void expand_units(const BasisPlan& bp, bool fam_a, int G, const uint2* q, int n, long long target,
+                         std::vector<uint2>& out) {
+    if (n <= 0) return;
+    long long total = 0;
+    for (int k = 0; k < n; ++k) total += (long long)bp.hdr[q[k].x].nprim * bp.hdr[q[k].y].nprim;
+    const long long U = target <= 0 ? 1 : fam_a ? target * 64 / G : target;
+    for (int k = 0; k < n; ++k) {
+        const int nb = bp.hdr[q[k].x].nprim, nk = bp.hdr[q[k].y].nprim;
+        const long long w = (long long)nb * nk;
+        long long S = 1;
+        if (target > 0) {
+            const long long w_min = fam_a ? 4LL * G : (long long)nk;
+            const long long W = std::max<long long>((total + U - 1) / U, std::max<long long>(w_min, 1));
+            S = (w + W - 1) / W;
+            const long long cap = fam_a ? std::max<long long>(1, w / (4LL * G)) : (long long)nb;
+            S = std::min<long long>(std::min<long long>(S, cap), SLICE_MAX);
+            if (S < 1) S = 1;
+        }
+        for (long long sl = 0; sl < S; ++sl)
+            out.push_back(make_uint2(q[k].x | ((unsigned)sl << 24), q[k].y | ((unsigned)S << 24)));
+    }
 }
 
 // Diagonal quartets (x|x) for the Schwarz bounds.

@@ -27,18 +27,31 @@ struct EmuCtx {
     std::vector<double> bounds, Qsh;
 };
 
-// Slices per quartet: 0 = the device's choice (choose_slices), > 0 = forced
-// (tests exercise the partial-epilogue path on small inputs).
+// Slicing of the emulated launches: the device's policy (expand_units with
+// SLICE_TARGET), or a forced slice count per quartet (tests exercise the
+// partial-epilogue path on small inputs), or none (0 target).
 int g_force_slices = 0;
 
-int emu_slices(bool fam_a, int G, int n, int maxbra, int maxwork, const EpiArgs& ep) {
-    if (ep.mode == EPI_DIAG) return 1;
-    return g_force_slices > 0 ? g_force_slices : choose_slices(fam_a, G, n, maxbra, maxwork);
+std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q, int n, const EpiArgs& ep) {
+    std::vector<uint2> out;
+    // the device slices J/K plans only (unique-array plans keep whole quartets)
+    if (ep.mode == EPI_DIAG || g_force_slices == 1 || (g_force_slices == 0 && ep.mode == EPI_UNIQUE)) {
+        out.assign(q, q + n);
+    } else if (g_force_slices > 1) {
+        for (int k = 0; k < n; ++k) {
+            const int cap = fam_a ? std::max(1, c.plan.hdr[q[k].x].nprim * c.plan.hdr[q[k].y].nprim / G)
+                                  : c.plan.hdr[q[k].x].nprim;
+            const int S = std::min(g_force_slices, cap);
+            for (int sl = 0; sl < S; ++sl) out.push_back(make_uint2(q[k].x | ((unsigned)sl << 24), q[k].y | ((unsigned)S << 24)));
+        }
+    } else {
+        expand_units(c.plan, fam_a, G, q, n, SLICE_TARGET, out);
+    }
+    return out;
 }
 
 template <int QC, class Real>
-void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucket, const int* maxbra,
-               const int* maxwork, const EpiArgs& ep0) {
+void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucket, const EpiArgs& ep) {
     using C = QClass<QC>;
     constexpr bool fam_a = use_family_a(C::LA, C::LB, C::LC, C::LD);
     const PrimPair<Real>* pp;
@@ -50,37 +63,32 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucke
         pp = c.plan.ppf.data();
         tab = c.boysf.data();
     }
-    if constexpr (fam_a) {
-        // replay the device's lane-group partition bucket by bucket
-        size_t pos = 0;
-        for (int g = 0; g < NGROUP; ++g) {
-            EpiArgs ep = ep0;
-            ep.split = emu_slices(true, GROUP_SIZE[g], bucket[g], maxbra[g], maxwork[g], ep0);
-            for (int k = 0; k < bucket[g]; ++k, ++pos)
-                family_a_group_emu<C::LA, C::LB, C::LC, C::LD, Real>(list[pos], GROUP_SIZE[g], c.plan.hdr.data(), pp,
-                                                                     tab, ep);
-        }
-    } else {
-        auto sm = std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
-        auto sync = []() {};
-        EpiArgs ep = ep0;
-        const int S = emu_slices(false, 64, (int)list.size(), maxbra[NGROUP - 1], maxwork[NGROUP - 1], ep0);
-        ep.split = S;
-        constexpr int NCH = FamBShape<C::LA, C::LB, C::LC, C::LD>::NCHUNK;
-        for (const uint2& e : list)
-            for (int sl = 0; sl < S; ++sl)
-                for (int ch = 0; ch < NCH; ++ch) {   // one device wavefront per (slice, chunk)
+    size_t pos = 0;
+    auto sm = fam_a ? nullptr : std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
+    for (int g = 0; g < NGROUP; ++g) {
+        const std::vector<uint2> units = emu_units(c, fam_a, GROUP_SIZE[g], list.data() + pos, bucket[g], ep);
+        pos += bucket[g];
+        for (const uint2& e : units) {
+            if constexpr (fam_a) {
+                // replay the device's lane-group partition unit by unit
+                family_a_group_emu<C::LA, C::LB, C::LC, C::LD, Real>(e, GROUP_SIZE[g], c.plan.hdr.data(), pp, tab, ep);
+            } else {
+                auto sync = []() {};
+                constexpr int NCH = FamBShape<C::LA, C::LB, C::LC, C::LD>::NCHUNK;
+                for (int ch = 0; ch < NCH; ++ch) {   // one device wavefront per (unit, chunk)
                     if (ep.mode == EPI_JK)
-                        family_b_one<C::LA, C::LB, C::LC, C::LD, Real, true>(e, sl, S, ch, c.plan.hdr.data(), pp, tab,
-                                                                             ep, *sm, c.ht, 0, 1, sync);
+                        family_b_one<C::LA, C::LB, C::LC, C::LD, Real, true>(e, ch, c.plan.hdr.data(), pp, tab, ep,
+                                                                             *sm, c.ht, 0, 1, sync);
                     else
-                        family_b_one<C::LA, C::LB, C::LC, C::LD, Real, false>(e, sl, S, ch, c.plan.hdr.data(), pp,
-                                                                              tab, ep, *sm, c.ht, 0, 1, sync);
+                        family_b_one<C::LA, C::LB, C::LC, C::LD, Real, false>(e, ch, c.plan.hdr.data(), pp, tab, ep,
+                                                                              *sm, c.ht, 0, 1, sync);
                 }
+            }
+        }
     }
 }
 
-using RunFn = void (*)(const EmuCtx&, const std::vector<uint2>&, const int*, const int*, const int*, const EpiArgs&);
+using RunFn = void (*)(const EmuCtx&, const std::vector<uint2>&, const int*, const EpiArgs&);
 
 template <class Real, int... QC>
 RunFn pick(std::integer_sequence<int, QC...>, int qc) {
@@ -91,9 +99,9 @@ RunFn pick(std::integer_sequence<int, QC...>, int qc) {
 void run_lists(const EmuCtx& c, const QuartetLists& ql, const EpiArgs& ep) {
     for (int qc = 0; qc < NQCLASS; ++qc) {
         if (!ql.list[qc][0].empty())
-            pick<double>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][0], ql.bucket[qc][0], ql.maxbra[qc][0], ql.maxwork[qc][0], ep);
+            pick<double>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][0], ql.bucket[qc][0], ep);
         if (!ql.list[qc][1].empty())
-            pick<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][1], ql.bucket[qc][1], ql.maxbra[qc][1], ql.maxwork[qc][1], ep);
+            pick<float>(std::make_integer_sequence<int, NQCLASS>{}, qc)(c, ql.list[qc][1], ql.bucket[qc][1], ep);
     }
 }
 
