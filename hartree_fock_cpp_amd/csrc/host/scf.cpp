@@ -9,6 +9,7 @@
 #include <stdexcept>
 
 #include "../../../include/hfx.h"
+#include "../device/scf_device.hpp"
 #include "linalg.hpp"
 
 namespace hfxh {
@@ -232,6 +233,48 @@ struct ScfCore {
     }
 };
 
+// Runs the SCF phases (the solve, then for the mixed form the FP64 polish)
+// on the host loop above or, when selected, on the GPU-resident iteration
+// (device/scf_device.hip); P, C and e of `core` hold the result either way.
+bool solve(ScfCore& core, Engine& eng, const ScfOptions& opt, int& n_it, int& n_polish, double& E_ele,
+           double& E_first) {
+    const bool device = opt.device_iteration > 0 || (opt.device_iteration < 0 && core.K >= hfxd::DEVSCF_MIN_K);
+    const bool polish = opt.precision == HFX_PREC_MIXED && opt.polish;
+    if (!device) {
+        bool ok = core.run(opt, opt.max_it, 4, true, n_it, E_ele);
+        E_first = E_ele;
+        if (ok && polish) {
+            eng.make_fock(opt, HFX_PREC_FP64);
+            ok = core.run(opt, opt.max_it, 1, false, n_polish, E_ele);
+        }
+        return ok;
+    }
+    hfxd::DevScf dev(core.K, core.nspin, core.nocc.data(), core.H.data(), core.S.data(), core.X.data());
+    for (int s = 0; s < core.nspin; ++s) dev.set_density(s, core.P[s].data());
+    dev.set_fock(eng.f);
+    hfxd::DevScfTimes dt;
+    bool ok = dev.run(opt.max_it, 4, true, opt.epsilon, opt.damping, opt.energy_convergence, opt.verbose, n_it, E_ele,
+                      dt);
+    E_first = E_ele;
+    if (ok && polish) {
+        eng.make_fock(opt, HFX_PREC_FP64);
+        dev.set_fock(eng.f);
+        ok = dev.run(opt.max_it, 1, false, opt.epsilon, opt.damping, opt.energy_convergence, opt.verbose, n_polish,
+                     E_ele, dt);
+    }
+    const size_t K2 = (size_t)core.K * core.K;
+    core.C.assign(core.nspin, std::vector<double>(K2));
+    core.e.assign(core.nspin, {});
+    for (int s = 0; s < core.nspin; ++s) {
+        dev.get_density(s, core.P[s].data());
+        dev.get_coefficients(s, core.C[s].data());
+        core.e[s] = dev.orbital_energies(s);
+    }
+    core.t->fock += dt.fock;
+    core.t->diag += dt.diag;
+    return ok;
+}
+
 struct Prepared {
     FlatBasis fb;
     BasisInfo info;
@@ -327,17 +370,12 @@ RhfResult rhf_diis(const Config& config, const std::string& basis_set, const std
         if (P0->size() != (size_t)K * K) throw std::runtime_error("initial density matrix dimension inconsistent with basis size");
         core.P[0] = *P0;
     }
-    int n_it = 0;
-    double E_ele = 0.0;
-    bool ok = core.run(opt, opt.max_it, 4, true, n_it, E_ele);
+    int n_it = 0, n2 = 0;
+    double E_ele = 0.0, E_first = 0.0;
+    const bool ok = solve(core, eng, opt, n_it, n2, E_ele, E_first);
     out.iterations = n_it;
-    out.E_before_polish = E_ele + p.E_nuc;
-    if (ok && opt.precision == HFX_PREC_MIXED && opt.polish) {
-        eng.make_fock(opt, HFX_PREC_FP64);
-        int n2 = 0;
-        ok = core.run(opt, opt.max_it, 1, false, n2, E_ele);
-        out.polish_iterations = n2;
-    }
+    out.polish_iterations = n2;
+    out.E_before_polish = E_first + p.E_nuc;
     out.converged = ok;
     out.E_ele = E_ele;
     out.E_nuc = p.E_nuc;
@@ -385,17 +423,12 @@ UhfResult uhf_diis(const Config& config, const std::string& basis_set, const std
         core.P[0] = *Pa0;
         core.P[1] = *Pb0;
     }
-    int n_it = 0;
-    double E_ele = 0.0;
-    bool ok = core.run(opt, opt.max_it, 4, true, n_it, E_ele);
+    int n_it = 0, n2 = 0;
+    double E_ele = 0.0, E_first = 0.0;
+    const bool ok = solve(core, eng, opt, n_it, n2, E_ele, E_first);
     out.iterations = n_it;
-    out.E_before_polish = E_ele + p.E_nuc;
-    if (ok && opt.precision == HFX_PREC_MIXED && opt.polish) {
-        eng.make_fock(opt, HFX_PREC_FP64);
-        int n2 = 0;
-        ok = core.run(opt, opt.max_it, 1, false, n2, E_ele);
-        out.polish_iterations = n2;
-    }
+    out.polish_iterations = n2;
+    out.E_before_polish = E_first + p.E_nuc;
     out.converged = ok;
     out.E_ele = E_ele;
     out.E_nuc = p.E_nuc;

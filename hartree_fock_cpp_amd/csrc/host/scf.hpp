@@ -36,6 +36,7 @@ struct ScfOptions {
     size_t hbm_budget = 0;
     bool polish = true;
     bool verbose = true;
+    int device_iteration = -1;     // 1 GPU-resident iteration, 0 host, -1 auto (K >= 96)
 };
 
 struct ScfTimings {

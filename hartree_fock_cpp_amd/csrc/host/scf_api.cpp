@@ -40,6 +40,7 @@ hfxh::ScfOptions to_opts(const hfx_scf_opts* o) {
     r.hbm_budget = o->hbm_budget;
     r.polish = o->polish != 0;
     r.verbose = o->verbose != 0;
+    r.device_iteration = o->device_iteration;
     return r;
 }
 
@@ -84,6 +85,7 @@ void hfx_scf_default_opts(hfx_scf_opts* o) {
     o->mixed_split = 1e-5;
     o->polish = 1;
     o->verbose = 0;
+    o->device_iteration = -1;
 }
 
 int hfx_basis_size(int natom, const char* symbols, const char* basis_set, const char* basis_path, int* K) {

@@ -116,6 +116,7 @@ struct hfx_server {
         o.hbm_budget = 0;
         o.polish = j->scf.polish != 0;
         o.verbose = false;
+        o.device_iteration = j->scf.device_iteration;
         try {
             auto fill = [&](auto& r, int Ne) {
                 hfx_scf_result& s = j->res.scf;

@@ -36,6 +36,8 @@ typedef struct {
     size_t hbm_budget;         /* 0 = unlimited */
     int polish;                /* FP64 polish after a mixed-precision solve */
     int verbose;
+    int device_iteration;      /* 1: SCF iteration resident on the GPU (Fock assembly, DIIS, transforms,
+                                  densities; host eigen-solve), 0: on the host, -1: auto (K >= 96) */
 } hfx_scf_opts;
 
 typedef struct {
