@@ -89,6 +89,8 @@ void dfree(T*& p) {
 
 namespace hfx_detail {
 void set_error(const std::string& msg) { g_err = msg; }
+// FP64 Boys table on the device (built on first use); nullptr with the error set on failure.
+const double* device_boys_table() { return ensure_consts() == 0 ? g_consts.boys_d : nullptr; }
 }  // namespace hfx_detail
 
 // Quartet lists of one plan, resident on the device, with per-class work counts.

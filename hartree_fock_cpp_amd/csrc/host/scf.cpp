@@ -296,11 +296,12 @@ Prepared prepare(const Config& config, const std::string& basis_set, const std::
     p.fb = basis_construction(config, basis_set, param_path, &p.info);
     const int K = p.fb.K;
     const size_t K2 = (size_t)K * K;
-    std::vector<double> T(K2), V(K2);
+    std::vector<double> T(K2), V(K2), Zd(Z.begin(), Z.end());
     p.S.assign(K2, 0.0);
-    kinetic_matrix(p.fb, T.data());
-    attraction_matrix(p.fb, (int)Z.size(), Z.data(), R.data(), V.data());
-    overlap_matrix(p.fb, p.S.data());
+    // S, T, V on the GPU (device/oneint_device.hip)
+    check(hfx_one_electron_device(K, p.fb.xyz.data(), p.fb.lmn.data(), p.fb.poff.data(), p.fb.alpha.data(),
+                                  p.fb.coeff.data(), p.fb.norm.data(), (int)Z.size(), Zd.data(), R.data(),
+                                  p.S.data(), T.data(), V.data()));
     p.H.resize(K2);
     for (size_t i = 0; i < K2; ++i) p.H[i] = T[i] + V[i];
     p.E_nuc = nuclear_repulsion(config);

@@ -84,6 +84,15 @@ int hfx_eri_unique_device(hfx_basis* b, double threshold, double* d_out, size_t 
  * G_ij = sum_kl P_kl [(ij|kl) - 1/2 (ik|jl)]. P, G: host K x K. */
 int hfx_repulsion_matrix(hfx_basis* b, const double* d_eri, const double* P, double* G);
 
+/* One-electron matrices on the GPU (K x K row-major host outputs; any may be
+ * NULL): S, T and V = -sum_C Z_C <a|1/|r-C||b> for the flat basis of
+ * hfx_basis_create and natom point charges Z at Rn (3*natom bohr).  Replaces
+ * the reference's host overlap_matrix / kinetic_matrix / attraction_matrix
+ * (include/integrals.hpp:99-118, 199-218, 516-536).  L <= 3. */
+int hfx_one_electron_device(int K, const double* xyz, const int* lmn, const int* poff, const double* alpha,
+                            const double* coeff, const double* norm, int natom, const double* Z, const double* Rn,
+                            double* S, double* T, double* V);
+
 /* Replaces coulomb_matrix (include/linalg.hpp:215-257) and exchange_matrix
  * (include/linalg.hpp:260-296) for a device-resident unique array, in one
  * pass: J from Pa + Pb, Ka from Pa, Kb from Pb (host K x K). */
