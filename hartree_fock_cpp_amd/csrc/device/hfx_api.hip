@@ -193,11 +193,14 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
     return 0;
 }
 
-// Wavefront target of the primitive-slice policy for J/K (direct Fock build)
-// plans: HFX_SLICE_TARGET in the environment when set (0 disables slicing),
-// else SLICE_TARGET.  Read once.  Unique-array plans are not sliced: their
-// slices would add through atomics into the output, and on the measured
-// builds (benzene/cc-pVDZ) whole quartets with plain stores were faster.
+// Wavefront target of the primitive-slice policy (expand_units):
+// HFX_SLICE_TARGET in the environment when set (0 disables slicing), else
+// SLICE_TARGET.  Read once.  Measured on benzene/cc-pVDZ and on small direct
+// builds (NaCl/cc-pVDZ): slicing family-B launches pays in every plan;
+// slicing the register-resident family-A lane groups pays in J/K plans
+// (launches of few quartets) but slowed the unique-array build, where the
+// strided primitive loop rebuilds the bra tables more often and the slices
+// add through atomics, so unique-array plans slice family B only.
 long long slice_target() {
     static const long long t = [] {
         const char* e = std::getenv("HFX_SLICE_TARGET");
@@ -206,10 +209,17 @@ long long slice_target() {
     return t;
 }
 
+struct SliceTargets {
+    long long fam_a, fam_b;
+};
+SliceTargets jk_slices() { return {slice_target(), slice_target()}; }
+SliceTargets unique_slices() { return {0, slice_target()}; }
+
 // Uploads the work units of the quartet lists (expand_units with the given
-// wavefront target; 0 for the Schwarz diagonal, whose epilogue needs whole
-// quartets) into one device buffer; fills offsets / counts / buckets in units.
-int upload_lists(const hfx_basis* b, const QuartetLists& ql, long long target, DevLists& dl) {
+// wavefront targets per family; 0 for the Schwarz diagonal, whose epilogue
+// needs whole quartets) into one device buffer; fills offsets / counts /
+// buckets in units.
+int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target, DevLists& dl) {
     std::vector<uint2> all;
     all.reserve(ql.total());
     for (int qc = 0; qc < NQCLASS; ++qc)
@@ -219,8 +229,8 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, long long target, D
             size_t pos = 0;
             for (int g = 0; g < NGROUP; ++g) {
                 const size_t before = all.size();
-                expand_units(b->plan, qc_family_a(qc), GROUP_SIZE[g], v.data() + pos, ql.bucket[qc][p][g], target,
-                             all);
+                expand_units(b->plan, qc_family_a(qc), GROUP_SIZE[g], v.data() + pos, ql.bucket[qc][p][g],
+                             qc_family_a(qc) ? target.fam_a : target.fam_b, all);
                 pos += ql.bucket[qc][p][g];
                 dl.bucket[qc][p][g] = (int)(all.size() - before);
             }
@@ -234,7 +244,7 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, long long target, D
 
 // Plans the screened quartets of a basis and uploads them (cached per basis
 // for the FP64 unique build).
-int make_plan(hfx_basis* b, double thr, int mixed, double split, long long target, DevLists& dl) {
+int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets target, DevLists& dl) {
     dfree(dl.d);
     dl.valid = false;
     QuartetLists ql;
@@ -270,7 +280,7 @@ int compute_bounds(hfx_basis* b) {
         });
     }
     DevLists dl;
-    if (int rc = upload_lists(b, ql, 0, dl)) return rc;
+    if (int rc = upload_lists(b, ql, SliceTargets{0, 0}, dl)) return rc;
     HIPCHK(launch_zero(b->d_bounds, P, b->stream));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_DIAG;
@@ -293,7 +303,7 @@ int eri_unique_into(hfx_basis* b, double threshold, double* d_out, hipStream_t s
     if (int rc = compute_bounds(b)) return rc;
     DevLists& dl = b->unique_plan;
     if (!dl.valid || dl.threshold != threshold)
-        if (int rc = make_plan(b, threshold, 0, 0.0, 0, dl)) return rc;
+        if (int rc = make_plan(b, threshold, 0, 0.0, unique_slices(), dl)) return rc;
     HIPCHK(launch_zero(d_out, n_unique(b->plan.K), st));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_UNIQUE;
@@ -473,7 +483,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     if (opts->precision == HFX_PREC_MIXED) mode = HFX_MODE_DIRECT;   // the mixed form is a direct build
     {
         const int mixed = opts->precision == HFX_PREC_MIXED;
-        if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split, slice_target(), f->plan)) {
+        if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split, jk_slices(), f->plan)) {
             hfx_fock_destroy(f);
             return rc;
         }

@@ -34,8 +34,8 @@ int g_force_slices = 0;
 
 std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q, int n, const EpiArgs& ep) {
     std::vector<uint2> out;
-    // the device slices J/K plans only (unique-array plans keep whole quartets)
-    if (ep.mode == EPI_DIAG || g_force_slices == 1 || (g_force_slices == 0 && ep.mode == EPI_UNIQUE)) {
+    // the device slices J/K plans, and family B only in unique-array plans
+    if (ep.mode == EPI_DIAG || g_force_slices == 1 || (g_force_slices == 0 && ep.mode == EPI_UNIQUE && fam_a)) {
         out.assign(q, q + n);
     } else if (g_force_slices > 1) {
         for (int k = 0; k < n; ++k) {
