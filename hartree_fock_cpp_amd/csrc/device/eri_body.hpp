@@ -581,7 +581,9 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
     const int ch_begin = chunk < 0 ? 0 : chunk, ch_end = chunk < 0 ? S::NCHUNK : chunk + 1;
     for (int ch = ch_begin; ch < ch_end; ++ch) {
         const int cd0 = ch * S::CDC;
-        const int ncd = (S::NCD - cd0) < S::CDC ? (S::NCD - cd0) : S::CDC;
+        // a compile-time count when the class has one chunk, so the item
+        // decodes below divide by a constant
+        const int ncd = S::NCHUNK == 1 ? S::NCD : ((S::NCD - cd0) < S::CDC ? (S::NCD - cd0) : S::CDC);
         if (ch > ch_begin) sync();   // the previous chunk's epilogue has read sm.acc
         for (int w = lane; w < S::NAB * S::CDC; w += nlanes) sm.acc[w] = Real(0);
         for (int i = slice; i < hb.nprim; i += nslice) {

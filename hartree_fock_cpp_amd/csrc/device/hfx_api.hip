@@ -5,6 +5,8 @@
 // status codes with a thread-local message, never as exit().
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
@@ -87,8 +89,11 @@ void dfree(T*& p) {
 
 }  // namespace
 
+thread_local bool t_no_graphs = false;   // see fock_graphs_on()
+
 namespace hfx_detail {
 void set_error(const std::string& msg) { g_err = msg; }
+void disable_graphs_in_this_thread() { t_no_graphs = true; }
 // FP64 Boys table on the device (built on first use); nullptr with the error set on failure.
 const double* device_boys_table() { return ensure_consts() == 0 ? g_consts.boys_d : nullptr; }
 }  // namespace hfx_detail
@@ -145,6 +150,9 @@ struct hfx_fock {
     DevLists plan;                           // direct mode: all lists back to back
     double *d_Dj = nullptr, *d_Da = nullptr, *d_Db = nullptr;
     double *d_J = nullptr, *d_Ka = nullptr, *d_Kb = nullptr;
+    // the build captured as a HIP graph for the last (densities, outputs, stream)
+    hipGraphExec_t graph = nullptr;
+    const void* graph_key[6] = {};
 };
 
 namespace {
@@ -526,6 +534,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
 
 void hfx_fock_destroy(hfx_fock* f) {
     if (!f) return;
+    if (f->graph) (void)hipGraphExecDestroy(f->graph);
     dfree(f->d_eri);
     dfree(f->plan.d);
     dfree(f->d_Dj);   // one allocation for all six matrices
@@ -549,20 +558,37 @@ int hfx_fock_class_counts(const hfx_fock* f, long long* n64, long long* n32) {
     return 0;
 }
 
-int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka,
-                          double* d_Kb, void* stream) {
-    if (!f || !d_Pa || !d_J || !d_Ka) return fail(HFX_ERR_INVALID, "null argument");
+namespace {
+
+// Graph replay of the Fock build: on by default, HFX_FOCK_GRAPH=0 in the
+// environment or hfx_set_fock_graphs(0) turns it off.  Threads that run
+// concurrently with other threads' synchronous HIP calls (the server's
+// workers) turn it off for themselves: a stream capture in flight makes the
+// runtime reject legacy-stream operations issued by other threads.
+std::atomic<int> g_fock_graphs{-1};
+bool fock_graphs_on() {
+    if (t_no_graphs) return false;
+    int v = g_fock_graphs.load();
+    if (v < 0) {
+        const char* e = std::getenv("HFX_FOCK_GRAPH");
+        v = (e && std::atoi(e) == 0) ? 0 : 1;
+        g_fock_graphs.store(v);
+    }
+    return v == 1;
+}
+
+// Enqueues one build on `st` (eager, or recorded under stream capture): only
+// kernel launches, no memset / memcpy nodes.
+int enqueue_build(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka, double* d_Kb,
+                  hipStream_t st) {
     hfx_basis* b = f->b;
-    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
     const int K = b->plan.K;
     const size_t K2 = (size_t)K * K;
     const int nspin = d_Pb ? 2 : 1;
-    if (nspin == 2 && !d_Kb) return fail(HFX_ERR_INVALID, "UHF build needs Kb");
     // Dj = Pa (+ Pb) for the Coulomb part
     const double* Dj = d_Pa;
     if (nspin == 2) {
-        HIPCHK(hipMemcpyAsync(f->d_Dj, d_Pa, K2 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        HIPCHK(launch_axpy(f->d_Dj, d_Pb, K2, st));
+        HIPCHK(launch_add(f->d_Dj, d_Pa, d_Pb, K2, st));
         Dj = f->d_Dj;
     }
     if (f->mode == HFX_MODE_STORED) return jk_from_unique(b, f->d_eri, Dj, d_Pa, d_Pb, d_J, d_Ka, d_Kb, nspin, st);
@@ -583,6 +609,46 @@ int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, d
     HIPCHK(launch_symmetrize(d_J, K, st));
     HIPCHK(launch_symmetrize(d_Ka, K, st));
     if (nspin == 2) HIPCHK(launch_symmetrize(d_Kb, K, st));
+    return 0;
+}
+
+}  // namespace
+
+int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka,
+                          double* d_Kb, void* stream) {
+    if (!f || !d_Pa || !d_J || !d_Ka) return fail(HFX_ERR_INVALID, "null argument");
+    if (d_Pb && !d_Kb) return fail(HFX_ERR_INVALID, "UHF build needs Kb");
+    hipStream_t st = stream ? (hipStream_t)stream : f->b->stream;
+    // class timing records events between launches: always eager; a stored
+    // build is a handful of launches: no graph
+    if (g_timer.on || f->mode == HFX_MODE_STORED || !fock_graphs_on() || !st)
+        return enqueue_build(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st);
+    const void* key[6] = {d_Pa, d_Pb, d_J, d_Ka, d_Kb, st};
+    if (!f->graph || !std::equal(key, key + 6, f->graph_key)) {
+        if (f->graph) {
+            HIPCHK(hipGraphExecDestroy(f->graph));
+            f->graph = nullptr;
+        }
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_build(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(e);
+        const hipError_t ei = hipGraphInstantiate(&f->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ei);
+        std::copy(key, key + 6, f->graph_key);
+    }
+    HIPCHK(hipGraphLaunch(f->graph, st));
+    return 0;
+}
+
+int hfx_set_fock_graphs(int on) {
+    g_fock_graphs.store(on ? 1 : 0);
     return 0;
 }
 

@@ -164,6 +164,21 @@ hipError_t launch_axpy(double* y, const double* x, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+// z = x + y (a kernel, not a memcpy + add, so the Fock build can be captured
+// into a graph with kernel nodes only)
+__global__ void k_add(double* z, const double* x, const double* y, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        z[i] = x[i] + y[i];
+}
+
+hipError_t launch_add(double* z, const double* x, const double* y, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_add<<<(unsigned)blocks, 256, 0, st>>>(z, x, y, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero(double* p, size_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;

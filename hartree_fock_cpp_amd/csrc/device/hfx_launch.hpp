@@ -26,5 +26,6 @@ hipError_t launch_jk_unique(const double* eri, const int2* pairs, int K, int nsp
 hipError_t launch_symmetrize(double* M, int K, hipStream_t st);
 hipError_t launch_zero(double* p, size_t n, hipStream_t st);
 hipError_t launch_axpy(double* y, const double* x, size_t n, hipStream_t st);   // y += x
+hipError_t launch_add(double* z, const double* x, const double* y, size_t n, hipStream_t st);   // z = x + y
 
 }  // namespace hfx

@@ -24,6 +24,7 @@
 
 namespace hfx_detail {
 void set_error(const std::string& msg);
+void disable_graphs_in_this_thread();
 }
 
 namespace {
@@ -155,6 +156,8 @@ struct hfx_server {
     }
 
     void worker() {
+        // jobs run side by side on their own threads: no stream capture here
+        hfx_detail::disable_graphs_in_this_thread();
         std::unique_lock<std::mutex> lk(m);
         for (;;) {
             Job* pick = nullptr;

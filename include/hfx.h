@@ -135,6 +135,14 @@ int hfx_fock_class_counts(const hfx_fock* f, long long* n_fp64, long long* n_fp3
  * returns the last build's per-launch milliseconds, ms[2*qc + p] with p = 0
  * for FP64 and 1 for FP32 (0 for classes not launched). */
 int hfx_set_class_timing(int on);
+
+/* Graph replay of the Fock build (no reference counterpart): each hfx_fock
+ * captures its build's kernel launches into a HIP graph for the last
+ * (densities, outputs, stream) it was called with and replays it while they
+ * stay the same (direct builds; stored builds are a few launches).  On by
+ * default (HFX_FOCK_GRAPH=0 in the environment turns it off); builds with
+ * class timing on, and builds from the server's worker threads, run eagerly. */
+int hfx_set_fock_graphs(int on);
 int hfx_class_times(float* ms);
 
 /* Screened shell-quartet counts and primitive-quartet work (sum over quartets
