@@ -25,6 +25,10 @@
 #include "eri_types.hpp"
 #include "md_core.hpp"
 
+#ifndef HFX_DENSE_R
+#define HFX_DENSE_R 1   // dense last R level in family B (L <= 7); 0 = packed only
+#endif
+
 namespace hfx {
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -417,14 +421,22 @@ struct FamBShape {
     static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
     static constexpr int NAB = NA * NB, NCD = NC * ND;
     static constexpr int NHAB = nherm(LAB), NR = nherm(L);
-    static constexpr int KB = L <= 8 ? 4 : 1;   // ket primitive pairs processed per step
+    // L <= 7: R lives in one dense cube per ket primitive pair, R3[(t D + u) D + v]
+    // with D = L + 1, updated in place level by level (orders high to low), so
+    // the ket contraction walks it with constant strides; two ket primitive
+    // pairs per step keep the LDS block small (resident wavefronts per CU are
+    // bounded by LDS).  Larger L: packed ping-pong levels, four (L = 8) or one
+    // pair per step.
+    static constexpr bool DENSE = HFX_DENSE_R && L <= 7;
+    static constexpr int D = L + 1, ND3 = DENSE ? D * D * D : 1, NRP = DENSE ? 1 : NR;
+    static constexpr int KB = DENSE ? 2 : (L <= 8 ? 4 : 1);   // ket primitive pairs per step
     // epilogue staging (doubles): per-function scales of the four shells, the
     // bra / ket function-pair bounds, the J densities over ab and cd, and per
     // spin the four exchange density blocks D_ac, D_ad, D_bc, D_bd
     static constexpr int O_FS = 0, O_BAB = NA + NB + NC + ND, O_BCD = O_BAB + NAB, O_DJAB = O_BCD + NCD,
                          O_DJCD = O_DJAB + NAB, O_DK = O_DJCD + NCD, NDK = (NA + NB) * (NC + ND),
                          NEPI = O_DK + 2 * NDK;
-    static constexpr int BUDGET = 5600 - (KB - 1) * 2 * NR - NEPI;   // doubles for g + acc per chunk
+    static constexpr int BUDGET = 5600 - (DENSE ? KB * ND3 : (KB - 1) * 2 * NR) - NEPI;   // doubles for g + acc
     static constexpr int CDC0 = BUDGET / (NHAB + NAB);
     static constexpr int CDC = CDC0 < NCD ? CDC0 : NCD;
     static constexpr int NCHUNK = (NCD + CDC - 1) / CDC;
@@ -440,7 +452,7 @@ struct HermTab {
 template <int LA, int LB, int LC, int LD, class Real>
 struct FamBSmem {
     using S = FamBShape<LA, LB, LC, LD>;
-    Real R[S::KB][2][S::NR];
+    Real R[S::KB][2][S::NRP];          // packed ping-pong levels (not S::DENSE)
     Real g[S::NHAB * S::CDC];
     Real acc[S::NAB * S::CDC];
     Real eb[S::EB];
@@ -452,6 +464,14 @@ struct FamBSmem {
     unsigned char abc[S::NAB][6];      // bra component pair -> (ax, ay, az, bx, by, bz)
     unsigned char cdc[S::NCD][6];      // ket component pair -> (cx, cy, cz, dx, dy, dz)
     double epi[S::NEPI];               // epilogue staging, per quartet (offsets in FamBShape)
+    Real R3[S::KB][S::ND3];            // dense R cube (S::DENSE)
+    // R recursion sources per packed Hermite index w (class constants):
+    // packed levels: idx1 | idx2 << 10 | coef << 20 | axis << 26 (axis 3: the base a[n]);
+    // dense cube: dense index | axis << 16 | coef << 18
+    int rsrc[S::NR];
+#ifdef HFX_LDS_PAD
+    double pad[HFX_LDS_PAD];
+#endif
 };
 
 // Stages what the epilogue of one quartet reads: function scales, pair bounds
@@ -530,6 +550,22 @@ HFX_HD __forceinline__ void famb_tables(FamBSmem<LA, LB, LC, LD, Real>& sm, cons
         sm.cdc[w][3] = (unsigned char)cart_x(LD, d);
         sm.cdc[w][4] = (unsigned char)cart_y(LD, d);
         sm.cdc[w][5] = (unsigned char)cart_z(LD, d);
+    }
+    // R_{tuv} = X R_{t-1,u,v} + (t-1) R_{t-2,u,v} (first non-zero index), base a[n] at (0,0,0)
+    for (int w = lane; w < S::NR; w += nlanes) {
+        const int t = ht.t[w], u = ht.u[w], v = ht.v[w];
+        int i1 = 0, i2 = 0, c = 0, ax = 3;
+        if (t > 0) {
+            ax = 0; i1 = hidx(t - 1, u, v); c = t - 1; i2 = t > 1 ? hidx(t - 2, u, v) : i1;
+        } else if (u > 0) {
+            ax = 1; i1 = hidx(t, u - 1, v); c = u - 1; i2 = u > 1 ? hidx(t, u - 2, v) : i1;
+        } else if (v > 0) {
+            ax = 2; i1 = hidx(t, u, v - 1); c = v - 1; i2 = v > 1 ? hidx(t, u, v - 2) : i1;
+        }
+        if constexpr (S::DENSE)
+            sm.rsrc[w] = ((t * S::D + u) * S::D + v) | (ax << 16) | (c << 18);
+        else
+            sm.rsrc[w] = i1 | (i2 << 10) | (c << 20) | (ax << 26);
     }
 }
 
@@ -625,35 +661,53 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
                 }
                 sync();
                 // R levels n = L .. 0 : level n holds orders <= L - n, ping-pong buffers
+                int cur = 0;
+                if constexpr (S::DENSE) {
+                    // in place: level n overwrites orders s = L-n .. 1 from high to
+                    // low (reading orders s-1, s-2 that still hold level n+1), then
+                    // the base R_000 = a[n]
+                    for (int kb = kb0; kb < nkb; kb += kbstep)
+                        if (gl == 0) sm.R3[kb][0] = sm.a[kb][S::L];
+                    constexpr int st[3] = {S::D * S::D, S::D, 1};
+                    for (int n = S::L - 1; n >= 0; --n) {
+                        for (int so = S::L - n; so >= 0; --so) {
+                            sync();
+                            const int w0 = so == 0 ? 0 : nherm(so - 1), w1 = nherm(so);
+                            for (int kb = kb0; kb < nkb; kb += kbstep) {
+                                Real* R = sm.R3[kb];
+                                for (int w = w0 + gl; w < w1; w += grp) {
+                                    const int src = sm.rsrc[w], dd = src & 0xFFFF, ax = (src >> 16) & 3;
+                                    if (ax == 3) {
+                                        R[0] = sm.a[kb][n];
+                                    } else {
+                                        const int sd = st[ax];
+                                        const int c = src >> 18;
+                                        Real val = sm.rx[kb][ax] * R[dd - sd];
+                                        if (c > 0) val += Real(c) * R[dd - 2 * sd];
+                                        R[dd] = val;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                } else {
+                // packed levels n = L .. 0 : level n holds orders <= L - n, ping-pong buffers
                 for (int kb = kb0; kb < nkb; kb += kbstep)
                     if (gl == 0) sm.R[kb][0][0] = sm.a[kb][S::L];
-                int cur = 0;
                 for (int n = S::L - 1; n >= 0; --n) {
                     sync();
                     const int cnt = nherm(S::L - n);
                     for (int kb = kb0; kb < nkb; kb += kbstep) {
                         const Real* Rp = sm.R[kb][cur];
                         Real* Rn = sm.R[kb][cur ^ 1];
-                        const Real rx = sm.rx[kb][0], ry = sm.rx[kb][1], rz = sm.rx[kb][2];
                         for (int w = gl; w < cnt; w += grp) {
-                            const int t = sm.ht[w][0], u = sm.ht[w][1], v = sm.ht[w][2];
-                            Real val;
-                            if (t > 0) {
-                                val = rx * Rp[hidx(t - 1, u, v)];
-                                if (t > 1) val += Real(t - 1) * Rp[hidx(t - 2, u, v)];
-                            } else if (u > 0) {
-                                val = ry * Rp[hidx(t, u - 1, v)];
-                                if (u > 1) val += Real(u - 1) * Rp[hidx(t, u - 2, v)];
-                            } else if (v > 0) {
-                                val = rz * Rp[hidx(t, u, v - 1)];
-                                if (v > 1) val += Real(v - 1) * Rp[hidx(t, u, v - 2)];
-                            } else {
-                                val = sm.a[kb][n];
-                            }
-                            Rn[w] = val;
+                            const int src = sm.rsrc[w], ax = src >> 26;
+                            Rn[w] = ax == 3 ? sm.a[kb][n]
+                                            : sm.rx[kb][ax] * Rp[src & 1023] + Real((src >> 20) & 63) * Rp[(src >> 10) & 1023];
                         }
                     }
                     cur ^= 1;
+                }
                 }
                 sync();
                 // stage 1: g[tuv][cd] += sum_kb sum Eket_cd(tau nu phi) R_kb(t+tau, u+nu, v+phi)
@@ -666,19 +720,40 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
                     const int oy = SK + (cy * (LD + 1) + dy) * (S::LCD + 1);
                     const int oz = 2 * SK + (cz * (LD + 1) + dz) * (S::LCD + 1);
                     Real s = Real(0);
-                    for (int kb = 0; kb < nkb; ++kb) {
-                        const Real* R = sm.R[kb][cur];
-                        const Real* Ex = sm.ek[kb] + ox;
-                        const Real* Ey = sm.ek[kb] + oy;
-                        const Real* Ez = sm.ek[kb] + oz;
-                        for (int ta = 0; ta <= cx + dx; ++ta) {
-                            Real sy = Real(0);
-                            for (int nn = 0; nn <= cy + dy; ++nn) {
-                                Real sz = Real(0);
-                                for (int ph = 0; ph <= cz + dz; ++ph) sz += Ez[ph] * R[hidx(t + ta, u + nn, v + ph)];
-                                sy += Ey[nn] * sz;
+                    if constexpr (S::DENSE) {
+                        const int b0 = (t * S::D + u) * S::D + v;
+                        const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+                        for (int kb = 0; kb < nkb; ++kb) {
+                            const Real* Ex = sm.ek[kb] + ox;
+                            const Real* Ey = sm.ek[kb] + oy;
+                            const Real* Ez = sm.ek[kb] + oz;
+                            const Real* Rt = sm.R3[kb] + b0;
+                            for (int ta = 0; ta <= nx; ++ta, Rt += S::D * S::D) {
+                                Real sy = Real(0);
+                                const Real* Ru = Rt;
+                                for (int nn = 0; nn <= ny; ++nn, Ru += S::D) {
+                                    Real sz = Real(0);
+                                    for (int ph = 0; ph <= nz; ++ph) sz += Ez[ph] * Ru[ph];
+                                    sy += Ey[nn] * sz;
+                                }
+                                s += Ex[ta] * sy;
                             }
-                            s += Ex[ta] * sy;
+                        }
+                    } else {
+                        for (int kb = 0; kb < nkb; ++kb) {
+                            const Real* R = sm.R[kb][cur];
+                            const Real* Ex = sm.ek[kb] + ox;
+                            const Real* Ey = sm.ek[kb] + oy;
+                            const Real* Ez = sm.ek[kb] + oz;
+                            for (int ta = 0; ta <= cx + dx; ++ta) {
+                                Real sy = Real(0);
+                                for (int nn = 0; nn <= cy + dy; ++nn) {
+                                    Real sz = Real(0);
+                                    for (int ph = 0; ph <= cz + dz; ++ph) sz += Ez[ph] * R[hidx(t + ta, u + nn, v + ph)];
+                                    sy += Ey[nn] * sz;
+                                }
+                                s += Ex[ta] * sy;
+                            }
                         }
                     }
                     sm.g[h * S::CDC + cdl] += s;
