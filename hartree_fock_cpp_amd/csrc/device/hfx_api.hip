@@ -91,6 +91,11 @@ void dfree(T*& p) {
 
 thread_local bool t_no_graphs = false;   // see fock_graphs_on()
 
+// Streams besides the caller's that the class launches of one plan spread
+// over: four launch queues in all, the number of hardware queues HIP gives a
+// process by default.
+constexpr int NSIDE = 3;
+
 namespace hfx_detail {
 void set_error(const std::string& msg) { g_err = msg; }
 void disable_graphs_in_this_thread() { t_no_graphs = true; }
@@ -125,6 +130,11 @@ struct hfx_basis {
     std::vector<double> Qsh;         // shell-pair maxima
     DevLists unique_plan;            // cached FP64 plan of the last unique-ERI build
     hipStream_t stream = nullptr;
+    // class launches of one plan fan out over the caller's stream and these
+    // (fork / join through events), so the small launches of one build overlap
+    hipStream_t side[NSIDE] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[NSIDE] = {};
 };
 
 namespace {
@@ -171,11 +181,48 @@ EpiArgs base_epi(const hfx_basis* b) {
     return ep;
 }
 
+// Relative cost of one list: kept primitive quartets times the work of one
+// (Hermite x Hermite plus Cartesian x Cartesian products of the class).
+double list_cost(const DevLists& dl, int qc, int p) {
+    const int pb = qc_bra(qc), pk = qc_ket(qc);
+    const int la = pc_la(pb), lb = pc_lb(pb), lc = pc_la(pk), ld = pc_lb(pk);
+    auto nc = [](int l) { return (l + 1) * (l + 2) / 2; };
+    auto nh = [](int l) { return (l + 1) * (l + 2) * (l + 3) / 6; };
+    return (double)dl.nprimq[qc][p] * (nh(la + lb) * nh(lc + ld) + nc(la) * nc(lb) * nc(lc) * nc(ld));
+}
+
 int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipStream_t st) {
-    for (int qc = 0; qc < NQCLASS; ++qc)
-        for (int p = 0; p < 2; ++p) {
-            g_timer.used[qc][p] = false;
-            if (dl.cnt[qc][p] == 0) continue;
+    // Lists go, largest first, to the least loaded of the caller's stream and
+    // the basis's side streams; the caller's stream then waits for the others.
+    // Class timing keeps every launch on the caller's stream.
+    const bool fan = !g_timer.on && b->ev_fork;
+    hipStream_t lane_st[NSIDE + 1] = {st};
+    for (int i = 0; i < NSIDE; ++i) lane_st[i + 1] = b->side[i];
+    int order[NQCLASS * 2], nl = 0;
+    for (int l = 0; l < 2 * NQCLASS; ++l)
+        if (dl.cnt[l / 2][l % 2]) order[nl++] = l;
+    if (fan)
+        std::stable_sort(order, order + nl, [&](int x, int y) {
+            return list_cost(dl, x / 2, x % 2) > list_cost(dl, y / 2, y % 2);
+        });
+    double load[NSIDE + 1] = {};
+    bool used_side[NSIDE] = {};
+    if (fan) HIPCHK(hipEventRecord(b->ev_fork, st));
+    for (int qc = 0; qc < NQCLASS; ++qc) g_timer.used[qc][0] = g_timer.used[qc][1] = false;
+    for (int k = 0; k < nl; ++k) {
+        const int qc = order[k] / 2, p = order[k] % 2;
+        int lane = 0;
+        if (fan) {
+            for (int i = 1; i <= NSIDE; ++i)
+                if (load[i] < load[lane]) lane = i;
+            load[lane] += list_cost(dl, qc, p) + 1.0;
+            if (lane > 0 && !used_side[lane - 1]) {
+                HIPCHK(hipStreamWaitEvent(lane_st[lane], b->ev_fork, 0));
+                used_side[lane - 1] = true;
+            }
+        }
+        hipStream_t ls = lane_st[lane];
+        {
             if (g_timer.on) HIPCHK(hipEventRecord(g_timer.ev[qc][p][0], st));
             size_t off = dl.off[qc][p];
             for (int g = 0; g < NGROUP; ++g) {
@@ -190,13 +237,19 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
                 L.herm = g_consts.herm;
                 L.G = GROUP_SIZE[g];
                 L.ep = ep;
-                HIPCHK(launch_eri_class(qc, p == 1, L, st));
+                HIPCHK(launch_eri_class(qc, p == 1, L, ls));
                 off += n;
             }
             if (g_timer.on) {
                 HIPCHK(hipEventRecord(g_timer.ev[qc][p][1], st));
                 g_timer.used[qc][p] = true;
             }
+        }
+    }
+    for (int i = 0; i < NSIDE; ++i)
+        if (used_side[i]) {
+            HIPCHK(hipEventRecord(b->ev_join[i], b->side[i]));
+            HIPCHK(hipStreamWaitEvent(st, b->ev_join[i], 0));
         }
     return 0;
 }
@@ -371,6 +424,11 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
     if (!rc) rc = dalloc(&b->d_fnpairs, (size_t)P);
     hipError_t e = hipSuccess;
     if (!rc) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int i = 0; i < NSIDE; ++i) {
+        if (!rc && e == hipSuccess) e = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
+        if (!rc && e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming);
+    }
+    if (!rc && e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
     if (!rc && e == hipSuccess) e = hipMemcpy(b->d_hdr, p.hdr.data(), p.hdr.size() * sizeof(PairHdr), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess && !p.ppd.empty())
         e = hipMemcpy(b->d_ppd, p.ppd.data(), p.ppd.size() * sizeof(PrimPair<double>), hipMemcpyHostToDevice);
@@ -397,6 +455,11 @@ void hfx_basis_destroy(hfx_basis* b) {
     dfree(b->d_fnpairs);
     dfree(b->unique_plan.d);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (int i = 0; i < NSIDE; ++i) {
+        if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+        if (b->ev_join[i]) (void)hipEventDestroy(b->ev_join[i]);
+    }
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     delete b;
 }
 
