@@ -413,6 +413,32 @@ inline void family_a_group_emu(uint2 e, int G, const PairHdr* hdr, const PrimPai
 #endif
 
 // ------------------------------------------------------------------ family B
+// Ket primitive pairs per step of a family-B class of total angular momentum L.
+__host__ __device__ constexpr int famb_kb(int L) { return (HFX_DENSE_R && L <= 7) ? 2 : (L <= 8 ? 4 : 1); }
+
+// Range of primitive pairs a family-B work unit covers.  Slice s of S with
+// S <= nb (bra primitive pairs): bra pairs s, s + S, ... against all ket
+// pairs.  S > nb (the planner then makes S a multiple of nb): bra pair s % nb
+// against ket chunk s / nb of S / nb chunks, each chunk a whole number of
+// steps (kb pairs).
+struct FamBRange {
+    int i0, istep, j0, j1;
+};
+__host__ __device__ inline FamBRange famb_range(int slice, int nslice, int nb, int nk, int kb) {
+    FamBRange r{slice, nslice, 0, nk};
+    if (nslice > nb) {
+        const int nkc = nslice / nb, kc = slice / nb;
+        const int per = (nk + nkc - 1) / nkc;
+        const int chunk = (per + kb - 1) / kb * kb;
+        r.i0 = slice % nb;
+        r.istep = nb;
+        r.j0 = kc * chunk;
+        r.j1 = r.j0 + chunk < nk ? r.j0 + chunk : nk;
+        if (kc >= nkc) r.i0 = nb;   // no work
+    }
+    return r;
+}
+
 // Shared-memory footprint is sized per class; the ket component pairs (cd)
 // are processed in chunks of CDC columns.
 template <int LA, int LB, int LC, int LD>
@@ -429,7 +455,7 @@ struct FamBShape {
     // pair per step.
     static constexpr bool DENSE = HFX_DENSE_R && L <= 7;
     static constexpr int D = L + 1, ND3 = DENSE ? D * D * D : 1, NRP = DENSE ? 1 : NR;
-    static constexpr int KB = DENSE ? 2 : (L <= 8 ? 4 : 1);   // ket primitive pairs per step
+    static constexpr int KB = famb_kb(L);   // ket primitive pairs per step
     // epilogue staging (doubles): per-function scales of the four shells, the
     // bra / ket function-pair bounds, the J densities over ab and cd, and per
     // spin the four exchange density blocks D_ac, D_ad, D_bc, D_bd
@@ -606,7 +632,8 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
     constexpr int SK = (LC + 1) * (LD + 1) * (S::LCD + 1);
     const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
     const int slice = unit_slice(e), nslice = unit_nslice(e);
-    if (slice >= hb.nprim) return;   // uniform over the workgroup: no bra primitives in this slice
+    const FamBRange rg = famb_range(slice, nslice, hb.nprim, hk.nprim, S::KB);
+    if (rg.i0 >= hb.nprim || rg.j0 >= rg.j1) return;   // uniform over the workgroup: an empty slice
     const QuartetCtx q = make_ctx(hb, hk);
     const double* fs = ep.fscale;
     const int K = ep.K;
@@ -622,7 +649,7 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
         const int ncd = S::NCHUNK == 1 ? S::NCD : ((S::NCD - cd0) < S::CDC ? (S::NCD - cd0) : S::CDC);
         if (ch > ch_begin) sync();   // the previous chunk's epilogue has read sm.acc
         for (int w = lane; w < S::NAB * S::CDC; w += nlanes) sm.acc[w] = Real(0);
-        for (int i = slice; i < hb.nprim; i += nslice) {
+        for (int i = rg.i0; i < hb.nprim; i += rg.istep) {
             const PrimPair<Real> b = pp[hb.prim_off + i];
             sync();
             build_e_lanes<LA, LB, Real>(sm.eb, b, false, lane, nlanes);
@@ -634,8 +661,8 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
             const int kb0 = nlanes >= S::KB ? lane / grp : 0;
             const int kbstep = nlanes >= S::KB ? S::KB : 1;
             const int gl = nlanes >= S::KB ? lane % grp : 0;
-            for (int j0 = 0; j0 < hk.nprim; j0 += S::KB) {
-                const int nkb = (hk.nprim - j0) < S::KB ? (hk.nprim - j0) : S::KB;
+            for (int j0 = rg.j0; j0 < rg.j1; j0 += S::KB) {
+                const int nkb = (rg.j1 - j0) < S::KB ? (rg.j1 - j0) : S::KB;
                 sync();
                 for (int kb = kb0; kb < nkb; kb += kbstep) {
                     const PrimPair<Real> k = pp[hk.prim_off + j0 + kb];

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../../include/hfx.h"
+#include "devmem.hpp"
 #include "hfx_launch.hpp"
 #include "plan.hpp"
 
@@ -70,20 +71,22 @@ int ensure_consts() {
     return 0;
 }
 
+// Per-job buffers come from the block cache (devmem.hpp): no device-wide
+// synchronization when a job's objects are released.
 template <class T>
 int dalloc(T** p, size_t n) {
     *p = nullptr;
     if (n == 0) return 0;
-    hipError_t e = hipMalloc(p, n * sizeof(T));
+    hipError_t e = hfx_detail::dev_alloc((void**)p, n * sizeof(T));
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? HFX_ERR_NOMEM : HFX_ERR_DEVICE,
-                    std::string("hipMalloc: ") + hipGetErrorString(e));
+                    std::string("device allocation: ") + hipGetErrorString(e));
     return 0;
 }
 
 template <class T>
 void dfree(T*& p) {
-    if (p) (void)hipFree(p);
+    if (p) hfx_detail::dev_free(p);
     p = nullptr;
 }
 
@@ -163,6 +166,8 @@ struct hfx_fock {
     // the build captured as a HIP graph for the last (densities, outputs, stream)
     hipGraphExec_t graph = nullptr;
     const void* graph_key[6] = {};
+    // recorded after every build on the build's stream: destroy waits for it
+    hipEvent_t ev_done = nullptr;
 };
 
 namespace {
@@ -195,7 +200,12 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
     // Lists go, largest first, to the least loaded of the caller's stream and
     // the basis's side streams; the caller's stream then waits for the others.
     // Class timing keeps every launch on the caller's stream.
-    const bool fan = !g_timer.on && b->ev_fork;
+    static const int nlaunch = [] {
+        const char* e = std::getenv("HFX_LAUNCH_STREAMS");
+        const int v = e ? std::atoi(e) : NSIDE + 1;
+        return v < 1 ? 1 : v > NSIDE + 1 ? NSIDE + 1 : v;
+    }();
+    const bool fan = !g_timer.on && b->ev_fork && nlaunch > 1;
     hipStream_t lane_st[NSIDE + 1] = {st};
     for (int i = 0; i < NSIDE; ++i) lane_st[i + 1] = b->side[i];
     int order[NQCLASS * 2], nl = 0;
@@ -213,7 +223,7 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
         const int qc = order[k] / 2, p = order[k] % 2;
         int lane = 0;
         if (fan) {
-            for (int i = 1; i <= NSIDE; ++i)
+            for (int i = 1; i < nlaunch; ++i)
                 if (load[i] < load[lane]) lane = i;
             load[lane] += list_cost(dl, qc, p) + 1.0;
             if (lane > 0 && !used_side[lane - 1]) {
@@ -291,7 +301,7 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target
             for (int g = 0; g < NGROUP; ++g) {
                 const size_t before = all.size();
                 expand_units(b->plan, qc_family_a(qc), GROUP_SIZE[g], v.data() + pos, ql.bucket[qc][p][g],
-                             qc_family_a(qc) ? target.fam_a : target.fam_b, all);
+                             qc_family_a(qc) ? target.fam_a : target.fam_b, all, qc_famb_kb(qc));
                 pos += ql.bucket[qc][p][g];
                 dl.bucket[qc][p][g] = (int)(all.size() - before);
             }
@@ -299,7 +309,10 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target
             dl.nquart[qc][p] = (long long)v.size();
         }
     if (int rc = dalloc(&dl.d, all.size())) return rc;
-    if (!all.empty()) HIPCHK(hipMemcpy(dl.d, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    if (!all.empty()) {
+        HIPCHK(hipMemcpyAsync(dl.d, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
     return 0;
 }
 
@@ -429,13 +442,15 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
         if (!rc && e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming);
     }
     if (!rc && e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
-    if (!rc && e == hipSuccess) e = hipMemcpy(b->d_hdr, p.hdr.data(), p.hdr.size() * sizeof(PairHdr), hipMemcpyHostToDevice);
-    if (!rc && e == hipSuccess && !p.ppd.empty())
-        e = hipMemcpy(b->d_ppd, p.ppd.data(), p.ppd.size() * sizeof(PrimPair<double>), hipMemcpyHostToDevice);
-    if (!rc && e == hipSuccess && !p.ppf.empty())
-        e = hipMemcpy(b->d_ppf, p.ppf.data(), p.ppf.size() * sizeof(PrimPair<float>), hipMemcpyHostToDevice);
-    if (!rc && e == hipSuccess) e = hipMemcpy(b->d_fscale, p.fscale.data(), K * sizeof(double), hipMemcpyHostToDevice);
-    if (!rc && e == hipSuccess) e = hipMemcpy(b->d_fnpairs, p.fnpairs.data(), P * sizeof(int2), hipMemcpyHostToDevice);
+    auto h2d = [&](void* dst, const void* src, size_t n) {
+        if (!rc && e == hipSuccess && n) e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, b->stream);
+    };
+    h2d(b->d_hdr, p.hdr.data(), p.hdr.size() * sizeof(PairHdr));
+    h2d(b->d_ppd, p.ppd.data(), p.ppd.size() * sizeof(PrimPair<double>));
+    h2d(b->d_ppf, p.ppf.data(), p.ppf.size() * sizeof(PrimPair<float>));
+    h2d(b->d_fscale, p.fscale.data(), K * sizeof(double));
+    h2d(b->d_fnpairs, p.fnpairs.data(), P * sizeof(int2));
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (!rc && e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("basis upload: ") + hipGetErrorString(e));
     if (rc) {
         hfx_basis_destroy(b);
@@ -447,6 +462,10 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
 
 void hfx_basis_destroy(hfx_basis* b) {
     if (!b) return;
+    // the basis' own streams: its buffers are back in the cache only once idle
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (int i = 0; i < NSIDE; ++i)
+        if (b->side[i]) (void)hipStreamSynchronize(b->side[i]);
     dfree(b->d_hdr);
     dfree(b->d_ppd);
     dfree(b->d_ppf);
@@ -491,7 +510,8 @@ int hfx_eri_unique(hfx_basis* b, double threshold, double* out, size_t Q) {
     if (int rc = dalloc(&d, Q)) return rc;
     int rc = eri_unique_into(b, threshold, d, b->stream);
     if (rc == 0) {
-        hipError_t e = hipMemcpy(out, d, Q * sizeof(double), hipMemcpyDeviceToHost);
+        hipError_t e = hipMemcpyAsync(out, d, Q * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("D2H: ") + hipGetErrorString(e));
     }
     dfree(d);
@@ -511,16 +531,16 @@ int hfx_coulomb_exchange(hfx_basis* b, const double* d_eri, const double* Pa, co
     if (int rc = dalloc(&buf, 6 * K2)) return rc;
     double *dDj = buf, *dDa = buf + K2, *dDb = buf + 2 * K2, *dJ = buf + 3 * K2, *dKa = buf + 4 * K2, *dKb = buf + 5 * K2;
     int rc = 0;
-    hipError_t e = hipMemcpy(dDj, Dj.data(), K2 * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dDa, Pa, K2 * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess && Pb) e = hipMemcpy(dDb, Pb, K2 * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(dDj, Dj.data(), K2 * sizeof(double), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dDa, Pa, K2 * sizeof(double), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess && Pb) e = hipMemcpyAsync(dDb, Pb, K2 * sizeof(double), hipMemcpyHostToDevice, b->stream);
     if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("H2D: ") + hipGetErrorString(e));
     if (!rc) rc = jk_from_unique(b, d_eri, dDj, dDa, dDb, dJ, dKa, dKb, nspin, b->stream);
     if (!rc) {
-        e = hipStreamSynchronize(b->stream);
-        if (e == hipSuccess && J) e = hipMemcpy(J, dJ, K2 * sizeof(double), hipMemcpyDeviceToHost);
-        if (e == hipSuccess && Ka) e = hipMemcpy(Ka, dKa, K2 * sizeof(double), hipMemcpyDeviceToHost);
-        if (e == hipSuccess && Kb && Pb) e = hipMemcpy(Kb, dKb, K2 * sizeof(double), hipMemcpyDeviceToHost);
+        if (J) e = hipMemcpyAsync(J, dJ, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess && Ka) e = hipMemcpyAsync(Ka, dKa, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess && Kb && Pb) e = hipMemcpyAsync(Kb, dKb, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("J/K: ") + hipGetErrorString(e));
     }
     dfree(buf);
@@ -575,6 +595,8 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     }
     double* buf = nullptr;
     int rc = dalloc(&buf, 6 * K2);
+    if (!rc && hipEventCreateWithFlags(&f->ev_done, hipEventDisableTiming) != hipSuccess)
+        rc = fail(HFX_ERR_DEVICE, "hipEventCreate");
     if (!rc) {
         f->d_Dj = buf;
         f->d_Da = buf + K2;
@@ -597,6 +619,11 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
 
 void hfx_fock_destroy(hfx_fock* f) {
     if (!f) return;
+    if (f->ev_done) {
+        (void)hipEventSynchronize(f->ev_done);
+        (void)hipEventDestroy(f->ev_done);
+    }
+    if (f->b && f->b->stream) (void)hipStreamSynchronize(f->b->stream);
     if (f->graph) (void)hipGraphExecDestroy(f->graph);
     dfree(f->d_eri);
     dfree(f->plan.d);
@@ -684,8 +711,11 @@ int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, d
     hipStream_t st = stream ? (hipStream_t)stream : f->b->stream;
     // class timing records events between launches: always eager; a stored
     // build is a handful of launches: no graph
-    if (g_timer.on || f->mode == HFX_MODE_STORED || !fock_graphs_on() || !st)
-        return enqueue_build(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st);
+    if (g_timer.on || f->mode == HFX_MODE_STORED || !fock_graphs_on() || !st) {
+        if (int rc = enqueue_build(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st)) return rc;
+        HIPCHK(hipEventRecord(f->ev_done, st));
+        return 0;
+    }
     const void* key[6] = {d_Pa, d_Pb, d_J, d_Ka, d_Kb, st};
     if (!f->graph || !std::equal(key, key + 6, f->graph_key)) {
         if (f->graph) {
@@ -707,6 +737,7 @@ int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, d
         std::copy(key, key + 6, f->graph_key);
     }
     HIPCHK(hipGraphLaunch(f->graph, st));
+    HIPCHK(hipEventRecord(f->ev_done, st));
     return 0;
 }
 

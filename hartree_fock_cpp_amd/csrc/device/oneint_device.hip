@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../../include/hfx.h"
+#include "devmem.hpp"
 #include "eri_body.hpp"
 
 namespace hfx_detail {
@@ -262,7 +263,7 @@ extern "C" int hfx_one_electron_device(int K, const double* xyz, const int* lmn,
     allp.reserve(off[16]);
     for (int c = 0; c < 16; ++c) allp.insert(allp.end(), pairs[c].begin(), pairs[c].end());
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&buf, o);
+    if (e == hipSuccess) e = hfx_detail::dev_alloc((void**)&buf, o);
     auto h2d = [&](size_t at, const void* src, size_t n) {
         if (e == hipSuccess && n) e = hipMemcpyAsync(buf + at, src, n, hipMemcpyHostToDevice, st);
     };
@@ -304,7 +305,7 @@ extern "C" int hfx_one_electron_device(int K, const double* xyz, const int* lmn,
     d2h(T, o_T);
     d2h(V, o_V);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (buf) (void)hipFree(buf);
+    hfx_detail::dev_free(buf);
     if (st) (void)hipStreamDestroy(st);
     if (e != hipSuccess) {
         hfx_detail::set_error(std::string("hfx_one_electron_device: ") + hipGetErrorString(e));

@@ -295,8 +295,14 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
 constexpr long long SLICE_TARGET = 2048;
 constexpr int SLICE_MAX = 255;
 
+// Ket primitive pairs per step of class qc's family-B kernel.
+inline int qc_famb_kb(int qc) {
+    const int pb = qc_bra(qc), pk = qc_ket(qc);
+    return famb_kb(pc_la(pb) + pc_lb(pb) + pc_la(pk) + pc_lb(pk));
+}
+
 inline void expand_units(const BasisPlan& bp, bool fam_a, int G, const uint2* q, int n, long long target,
-                         std::vector<uint2>& out) {
+                         std::vector<uint2>& out, int kb = 1) {
     if (n <= 0) return;
     long long total = 0, wmax = 0;
     for (int k = 0; k < n; ++k) {
@@ -310,20 +316,34 @@ inline void expand_units(const BasisPlan& bp, bool fam_a, int G, const uint2* q,
         out.insert(out.end(), q, q + n);
         return;
     }
+    auto emit = [&](const uint2& e, long long S) {
+        if (S <= 1) {
+            out.push_back(e);
+            return;
+        }
+        for (long long sl = 0; sl < S; ++sl)
+            out.push_back(make_uint2(e.x | ((unsigned)sl << 24), e.y | ((unsigned)S << 24)));
+    };
     for (int k = 0; k < n; ++k) {
         const int nb = bp.hdr[q[k].x].nprim, nk = bp.hdr[q[k].y].nprim;
         const long long w = (long long)nb * nk;
-        const long long w_min = fam_a ? 4LL * G : (long long)nk;
-        const long long Wq = std::max<long long>(W, std::max<long long>(w_min, 1));
-        long long S = (w + Wq - 1) / Wq;
-        const long long cap = fam_a ? std::max<long long>(1, w / (4LL * G)) : (long long)nb;
-        S = std::max<long long>(1, std::min<long long>(std::min<long long>(S, cap), SLICE_MAX));
-        if (S == 1) {
-            out.push_back(q[k]);
+        if (fam_a) {
+            const long long Wq = std::max<long long>(W, 4LL * G);
+            const long long cap = std::max<long long>(1, w / (4LL * G));
+            emit(q[k], std::max<long long>(1, std::min<long long>(std::min<long long>((w + Wq - 1) / Wq, cap), SLICE_MAX)));
             continue;
         }
-        for (long long sl = 0; sl < S; ++sl)
-            out.push_back(make_uint2(q[k].x | ((unsigned)sl << 24), q[k].y | ((unsigned)S << 24)));
+        // family B: whole ket loops per bra pair while a slot's share covers
+        // one; below that, ket chunks of at least two steps (famb_range)
+        if (W >= nk || nk <= 2 * kb) {
+            const long long Wq = std::max<long long>(W, nk);
+            emit(q[k], std::max<long long>(1, std::min<long long>(std::min<long long>((w + Wq - 1) / Wq, nb), SLICE_MAX)));
+            continue;
+        }
+        long long chunk = std::max<long long>(2 * kb, (W + kb - 1) / kb * kb);
+        long long nkc = (nk + chunk - 1) / chunk;
+        while (nkc > 1 && (long long)nb * nkc > SLICE_MAX) nkc = (nkc + 1) / 2;
+        emit(q[k], nkc > 1 ? nb * nkc : std::min<long long>(nb, SLICE_MAX));
     }
 }
 

@@ -14,6 +14,7 @@
 #include <string>
 
 #include "../host/linalg.hpp"
+#include "devmem.hpp"
 #include "scf_device.hpp"
 
 namespace hfxd {
@@ -266,7 +267,7 @@ DevScf::DevScf(int K, int nspin, const int* nocc, const double* H, const double*
     d.occ = nspin == 1 ? 2.0 : 1.0;
     e_.assign(nspin, std::vector<double>(K, 0.0));
     const size_t K2 = d.K2, ns = (size_t)nspin;
-    auto alloc = [&](double** p, size_t n) { hchk(hipMalloc(p, n * sizeof(double)), "hipMalloc"); };
+    auto alloc = [&](double** p, size_t n) { hchk(hfx_detail::dev_alloc((void**)p, n * sizeof(double)), "device allocation"); };
     try {
         hchk(hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking), "stream");
         alloc(&d.H, K2);
@@ -287,9 +288,9 @@ DevScf::DevScf(int K, int nspin, const int* nocc, const double* H, const double*
         alloc(&d.T2, K2);
         alloc(&d.sc, SC_N);
         alloc(&d.coef, MAXH);
-        hchk(hipHostMalloc(&d.h_sc, SC_N * sizeof(double)), "hipHostMalloc");
-        hchk(hipHostMalloc(&d.h_mat, K2 * sizeof(double)), "hipHostMalloc");
-        hchk(hipHostMalloc(&d.h_coef, MAXH * sizeof(double)), "hipHostMalloc");
+        hchk(hfx_detail::host_alloc((void**)&d.h_sc, SC_N * sizeof(double)), "pinned allocation");
+        hchk(hfx_detail::host_alloc((void**)&d.h_mat, K2 * sizeof(double)), "pinned allocation");
+        hchk(hfx_detail::host_alloc((void**)&d.h_coef, MAXH * sizeof(double)), "pinned allocation");
         hchk(hipMemcpyAsync(d.H, H, K2 * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
         hchk(hipMemcpyAsync(d.S, S, K2 * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
         hchk(hipMemcpyAsync(d.X, X, K2 * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
@@ -309,9 +310,9 @@ void DevScf::release() {
     if (d.st) (void)hipStreamSynchronize(d.st);
     for (double* q : {d.H, d.S, d.X, d.P, d.Pn, d.C, d.J, d.Ka, d.Kb, d.hF, d.hR, d.Fuse, d.Fc, d.Rc, d.T1, d.T2, d.sc,
                       d.coef})
-        if (q) (void)hipFree(q);
+        hfx_detail::dev_free(q);
     for (double* q : {d.h_sc, d.h_mat, d.h_coef})
-        if (q) (void)hipHostFree(q);
+        hfx_detail::host_free(q);
     if (d.st) (void)hipStreamDestroy(d.st);
     delete p_;
     p_ = nullptr;

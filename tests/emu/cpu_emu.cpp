@@ -29,14 +29,29 @@ struct EmuCtx {
 
 // Slicing of the emulated launches: the device's policy (expand_units with
 // SLICE_TARGET), or a forced slice count per quartet (tests exercise the
-// partial-epilogue path on small inputs), or none (0 target).
+// partial-epilogue path on small inputs), or none (0 target).  A negative
+// count -c cuts family-B quartets into c ket chunks per bra primitive pair
+// (famb_range), family A as for c.
 int g_force_slices = 0;
 
-std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q, int n, const EpiArgs& ep) {
+std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q, int n, const EpiArgs& ep, int kb) {
     std::vector<uint2> out;
     // the device slices J/K plans, and family B only in unique-array plans
     if (ep.mode == EPI_DIAG || g_force_slices == 1 || (g_force_slices == 0 && ep.mode == EPI_UNIQUE && fam_a)) {
         out.assign(q, q + n);
+    } else if (g_force_slices < 0) {
+        for (int k = 0; k < n; ++k) {
+            const int nb = c.plan.hdr[q[k].x].nprim, nk = c.plan.hdr[q[k].y].nprim;
+            int S;
+            if (fam_a) {
+                S = std::min(-g_force_slices, std::max(1, nb * nk / G));
+            } else {
+                int nkc = std::min(-g_force_slices, (nk + kb - 1) / kb);
+                while (nkc > 1 && nb * nkc > SLICE_MAX) --nkc;
+                S = nkc > 1 ? nb * nkc : 1;
+            }
+            for (int sl = 0; sl < S; ++sl) out.push_back(make_uint2(q[k].x | ((unsigned)sl << 24), q[k].y | ((unsigned)S << 24)));
+        }
     } else if (g_force_slices > 1) {
         for (int k = 0; k < n; ++k) {
             const int cap = fam_a ? std::max(1, c.plan.hdr[q[k].x].nprim * c.plan.hdr[q[k].y].nprim / G)
@@ -45,7 +60,7 @@ std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q,
             for (int sl = 0; sl < S; ++sl) out.push_back(make_uint2(q[k].x | ((unsigned)sl << 24), q[k].y | ((unsigned)S << 24)));
         }
     } else {
-        expand_units(c.plan, fam_a, G, q, n, SLICE_TARGET, out);
+        expand_units(c.plan, fam_a, G, q, n, SLICE_TARGET, out, kb);
     }
     return out;
 }
@@ -66,7 +81,8 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucke
     size_t pos = 0;
     auto sm = fam_a ? nullptr : std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
     for (int g = 0; g < NGROUP; ++g) {
-        const std::vector<uint2> units = emu_units(c, fam_a, GROUP_SIZE[g], list.data() + pos, bucket[g], ep);
+        const std::vector<uint2> units = emu_units(c, fam_a, GROUP_SIZE[g], list.data() + pos, bucket[g], ep,
+                                                   famb_kb(C::LA + C::LB + C::LC + C::LD));
         pos += bucket[g];
         for (const uint2& e : units) {
             if constexpr (fam_a) {

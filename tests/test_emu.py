@@ -148,10 +148,11 @@ def test_emu_mixed_close_to_fp64():
     assert n32.sum() > 0 and n64.sum() > 0
 
 
-@pytest.mark.parametrize("S", [1, 3, 7])
+@pytest.mark.parametrize("S", [1, 3, 7, -2, -5])
 def test_emu_primitive_slices_are_exact(S):
     """Slicing a quartet's primitive loop and adding the partial blocks (unique
-    array and J/K) reproduces the unsliced result to rounding."""
+    array and J/K) reproduces the unsliced result to rounding.  Negative S:
+    family-B quartets cut into |S| ket-primitive chunks per bra pair."""
     d = golden("h2o_ccpvdz.npz")
     K = int(d["K"])
     thr = float(d["threshold"])
