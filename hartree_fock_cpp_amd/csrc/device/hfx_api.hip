@@ -373,11 +373,8 @@ int compute_bounds(hfx_basis* b) {
     return 0;
 }
 
-int eri_unique_into(hfx_basis* b, double threshold, double* d_out, hipStream_t st) {
-    if (int rc = compute_bounds(b)) return rc;
-    DevLists& dl = b->unique_plan;
-    if (!dl.valid || dl.threshold != threshold)
-        if (int rc = make_plan(b, threshold, 0, 0.0, unique_slices(), dl)) return rc;
+// The unique array of a plan (FP64, or mixed: FP32 for its small-bound lists).
+int eri_unique_plan(hfx_basis* b, const DevLists& dl, double threshold, double* d_out, hipStream_t st) {
     HIPCHK(launch_zero(d_out, n_unique(b->plan.K), st));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_UNIQUE;
@@ -386,6 +383,14 @@ int eri_unique_into(hfx_basis* b, double threshold, double* d_out, hipStream_t s
     if (int rc = launch_plan(b, dl, ep, st)) return rc;
     HIPCHK(hipStreamSynchronize(st));
     return 0;
+}
+
+int eri_unique_into(hfx_basis* b, double threshold, double* d_out, hipStream_t st) {
+    if (int rc = compute_bounds(b)) return rc;
+    DevLists& dl = b->unique_plan;
+    if (!dl.valid || dl.threshold != threshold)
+        if (int rc = make_plan(b, threshold, 0, 0.0, unique_slices(), dl)) return rc;
+    return eri_unique_plan(b, dl, threshold, d_out, st);
 }
 
 // J/K from a device unique array into device accumulators, then symmetrised.
@@ -570,25 +575,23 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     f->opts = *opts;
     if (f->opts.mixed_split <= 0.0) f->opts.mixed_split = 1e-5;
     const size_t mats = 6 * K2 * sizeof(double);
-    int mode = opts->mode;
-    if (opts->precision == HFX_PREC_MIXED) mode = HFX_MODE_DIRECT;   // the mixed form is a direct build
-    {
-        const int mixed = opts->precision == HFX_PREC_MIXED;
-        if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split, jk_slices(), f->plan)) {
-            hfx_fock_destroy(f);
-            return rc;
-        }
-        size_t n = 0;
-        for (int qc = 0; qc < NQCLASS; ++qc) n += f->plan.nquart[qc][0] + f->plan.nquart[qc][1];
-        f->nq = n;
-    }
     const size_t stored_bytes = Q * sizeof(double) + mats;
-    const size_t direct_bytes = f->nq * sizeof(uint2) + mats;
+    const int mixed = opts->precision == HFX_PREC_MIXED;
+    int mode = opts->mode;
     if (mode == HFX_MODE_AUTO)
         mode = (opts->hbm_budget == 0 || stored_bytes <= opts->hbm_budget) ? HFX_MODE_STORED : HFX_MODE_DIRECT;
+    // direct: the J/K plan; stored mixed: the plan of its unique array (FP32
+    // for the small-bound quartets); stored FP64 only counts quartets here and
+    // builds from the basis' cached unique plan
+    if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split,
+                           mode == HFX_MODE_STORED ? unique_slices() : jk_slices(), f->plan)) {
+        hfx_fock_destroy(f);
+        return rc;
+    }
+    for (int qc = 0; qc < NQCLASS; ++qc) f->nq += f->plan.nquart[qc][0] + f->plan.nquart[qc][1];
+    const size_t direct_bytes = f->nq * sizeof(uint2) + mats;
     f->mode = mode;
     f->bytes = mode == HFX_MODE_STORED ? stored_bytes : direct_bytes;
-    if (mode == HFX_MODE_STORED) dfree(f->plan.d);   // stored builds use the basis' cached unique plan
     if (opts->hbm_budget && f->bytes > opts->hbm_budget) {
         hfx_fock_destroy(f);
         return fail(HFX_ERR_NOMEM, "job does not fit its HBM budget");
@@ -607,7 +610,9 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     }
     if (!rc && mode == HFX_MODE_STORED) {
         rc = dalloc(&f->d_eri, Q);
-        if (!rc) rc = eri_unique_into(b, opts->threshold, f->d_eri, b->stream);
+        if (!rc) rc = mixed ? eri_unique_plan(b, f->plan, opts->threshold, f->d_eri, b->stream)
+                            : eri_unique_into(b, opts->threshold, f->d_eri, b->stream);
+        dfree(f->plan.d);
     }
     if (rc) {
         hfx_fock_destroy(f);

@@ -87,8 +87,9 @@ struct hfx_server {
     // when it cannot start now.
     bool admissible(Job* j, size_t free_bytes, bool unlimited) {
         const int forced = j->scf.eri_mode;
-        // the mixed form is a direct build; FP64 prefers stored ERIs when they fit
-        const bool may_store = forced != HFX_MODE_DIRECT && j->scf.precision == HFX_PREC_FP64;
+        // both forms prefer stored ERIs when they fit (the mixed form then
+        // stores its FP32-evaluated small-bound integrals in the FP64 array)
+        const bool may_store = forced != HFX_MODE_DIRECT;
         if (may_store && (unlimited || j->bytes_stored <= free_bytes)) {
             j->mode = HFX_MODE_STORED;
             j->charged = j->bytes_stored;

@@ -108,7 +108,12 @@ typedef struct {
 } hfx_fock_opts;
 
 /* Fock-build context for one SCF job: owns its stored ERIs (stored mode) or
- * quartet lists (direct mode), device densities and accumulators. */
+ * quartet lists (direct mode), device densities and accumulators.  Mixed
+ * precision: direct builds evaluate and accumulate the small-bound quartets in
+ * FP32 every build; stored mode evaluates them in FP32 once and keeps them in
+ * the FP64 unique array.  AUTO picks stored when the unique array fits
+ * hbm_budget (0: always), for both precisions.  Destroying the context waits
+ * for its last build. */
 int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out);
 void hfx_fock_destroy(hfx_fock* f);
 

@@ -68,3 +68,27 @@ def test_mixed_direct_close_to_fp64(gpu):
     dJ, dK = np.abs(J1 - J0).max(), np.abs(K1 - K0).max()
     print(f"mixed vs fp64: fp32 quartets {n32.sum()} of {n32.sum() + n64.sum()}, max|dJ|={dJ:.2e} max|dK|={dK:.2e}")
     assert dJ < 1e-6 and dK < 1e-6
+
+
+def test_mixed_stored_close_to_fp64(gpu):
+    """Stored mode of the mixed form: the unique array holds FP32-evaluated
+    small-bound quartets; J/K within the mixed form's error of FP64 and of the
+    direct mixed build."""
+    d = golden("h2o_ccpvdz.npz")
+    K = int(d["K"])
+    b = gpu.Basis.from_arrays(d)
+    P = _dens(K, 4)
+    f64 = gpu.FockBuilder(b, mode="stored", precision="fp64", threshold=1e-12)
+    mst = gpu.FockBuilder(b, mode="stored", precision="mixed", threshold=1e-12, mixed_split=1e-3)
+    mdi = gpu.FockBuilder(b, mode="direct", precision="mixed", threshold=1e-12, mixed_split=1e-3)
+    assert mst.info()["mode"] == "stored"
+    n64, n32 = mst.class_counts()
+    assert n32.sum() > 0
+    J0, K0 = f64.build(P)
+    J1, K1 = mst.build(P)
+    J2, K2 = mdi.build(P)
+    print(f"stored mixed vs fp64: max|dJ|={np.abs(J1 - J0).max():.2e} max|dK|={np.abs(K1 - K0).max():.2e}; "
+          f"vs direct mixed: {np.abs(J1 - J2).max():.2e} {np.abs(K1 - K2).max():.2e}")
+    assert np.abs(J1 - J0).max() < 1e-6 and np.abs(K1 - K0).max() < 1e-6
+    assert np.abs(J1 - J2).max() < 1e-6 and np.abs(K1 - K2).max() < 1e-6
+    assert np.abs(J1 - J0).max() > 0.0   # the FP32 quartets were really evaluated in FP32
