@@ -47,7 +47,8 @@ lib.hfx_device_count.restype = C.c_int
 
 class FockOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("precision", C.c_int), ("threshold", C.c_double),
-                ("mixed_split", C.c_double), ("hbm_budget", C.c_size_t)]
+                ("mixed_split", C.c_double), ("hbm_budget", C.c_size_t), ("incremental", C.c_int),
+                ("incremental_tau", C.c_double)]
 
 
 def check(rc):

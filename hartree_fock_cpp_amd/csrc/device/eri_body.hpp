@@ -343,6 +343,18 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
     }
 }
 
+// True when the density-difference bound of a unit is below ep.dthr (the J/K
+// terms of (ab|cd) read the density blocks cd, ab, bd, bc, ad and ac).
+HFX_HD __forceinline__ bool dens_skip(const EpiArgs& ep, uint2 e, const PairHdr& hb, const PairHdr& hk) {
+    if (ep.dpmax == nullptr) return false;
+    const int n = ep.nshell;
+    const double* m = ep.dpmax;
+    const int a = hb.s1, b = hb.s2, c = hk.s1, d = hk.s2;
+    const double x = fmax(fmax(fmax(m[c * n + d], m[a * n + b]), fmax(m[b * n + d], m[b * n + c])),
+                          fmax(m[a * n + d], m[a * n + c]));
+    return ep.qpair[unit_bra(e)] * ep.qpair[unit_ket(e)] * x < ep.dthr;
+}
+
 HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk) {
     QuartetCtx q;
     q.fn1 = hb.fn1; q.fn2 = hb.fn2; q.fn3 = hk.fn1; q.fn4 = hk.fn2;
@@ -366,7 +378,7 @@ HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const 
     Real acc[F::NOUT];
 #pragma unroll
     for (int i = 0; i < F::NOUT; ++i) acc[i] = Real(0);
-    const bool work = active && slice * G < hb.nprim * hk.nprim;
+    const bool work = active && slice * G < hb.nprim * hk.nprim && !dens_skip(ep, e, hb, hk);
     if (work) F::quartet(hb, hk, pp, tab, acc, slice * G + sub, G * nslice);
 #if defined(__HIP_DEVICE_COMPILE__)
     if constexpr (G > 1) {
@@ -398,7 +410,7 @@ inline void family_a_group_emu(uint2 e, int G, const PairHdr* hdr, const PrimPai
     using F = FamA<LA, LB, LC, LD, Real>;
     const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
     const int slice = unit_slice(e), nslice = unit_nslice(e);
-    if (slice * G >= hb.nprim * hk.nprim) return;
+    if (slice * G >= hb.nprim * hk.nprim || dens_skip(ep, e, hb, hk)) return;
     Real tot[F::NOUT] = {};
     for (int sub = 0; sub < G; ++sub) {
         Real acc[F::NOUT] = {};
@@ -633,7 +645,8 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
     const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
     const int slice = unit_slice(e), nslice = unit_nslice(e);
     const FamBRange rg = famb_range(slice, nslice, hb.nprim, hk.nprim, S::KB);
-    if (rg.i0 >= hb.nprim || rg.j0 >= rg.j1) return;   // uniform over the workgroup: an empty slice
+    // uniform over the workgroup: an empty slice, or screened by the density difference
+    if (rg.i0 >= hb.nprim || rg.j0 >= rg.j1 || dens_skip(ep, e, hb, hk)) return;
     const QuartetCtx q = make_ctx(hb, hk);
     const double* fs = ep.fscale;
     const int K = ep.K;

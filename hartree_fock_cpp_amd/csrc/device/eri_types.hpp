@@ -55,6 +55,13 @@ struct EpiArgs {
     double* J;                 // J accumulator (symmetrised afterwards)
     double* Ka;                // K accumulators
     double* Kb;
+    // density-difference screening of incremental J/K builds: a unit is
+    // skipped when qpair[bra] * qpair[ket] * (max |dP| over the six shell
+    // blocks its J/K terms read) < dthr.  dpmax == nullptr: no screening.
+    const double* qpair;       // shell-pair Schwarz maxima by pair id
+    const double* dpmax;       // nshell x nshell block maxima of |dP|
+    int nshell;
+    double dthr;
 };
 
 // A work unit of a launch list: a shell quartet (bra, ket pair ids < 2^24)

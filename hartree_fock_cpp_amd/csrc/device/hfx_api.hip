@@ -128,6 +128,8 @@ struct hfx_basis {
     double* d_fscale = nullptr;
     double* d_bounds = nullptr;      // function-pair Schwarz bounds (P)
     int2* d_fnpairs = nullptr;       // function pair id -> (i, j)
+    int2* d_shells = nullptr;        // shell -> (first function, function count)
+    double* d_qpair = nullptr;       // shell-pair Schwarz maxima by pair id (with the bounds)
     bool bounds_ready = false;
     std::vector<double> bounds;      // host copy
     std::vector<double> Qsh;         // shell-pair maxima
@@ -168,6 +170,12 @@ struct hfx_fock {
     const void* graph_key[6] = {};
     // recorded after every build on the build's stream: destroy waits for it
     hipEvent_t ev_done = nullptr;
+    // incremental direct builds: previous densities, density differences,
+    // accumulated (symmetrised) J / K, shell-block maxima of |dP|
+    double* d_inc = nullptr;
+    bool have_prev = false;
+    int prev_nspin = 0;
+    int since_full = 0;
 };
 
 namespace {
@@ -184,6 +192,18 @@ EpiArgs base_epi(const hfx_basis* b) {
     ep.fscale = b->d_fscale;
     ep.nspin = 1;
     return ep;
+}
+
+// Screening threshold of the density-difference terms of incremental builds:
+// HFX_INCREMENTAL_TAU in the environment when set, else the option, else
+// 1e-3 x the Schwarz threshold (the skipped terms of one build then stay three
+// orders below what the Schwarz screen itself drops).
+double incremental_tau(const hfx_fock* f) {
+    static const double env = [] {
+        const char* e = std::getenv("HFX_INCREMENTAL_TAU");
+        return e ? std::atof(e) : -1.0;
+    }();
+    return env >= 0.0 ? env : f->opts.incremental_tau > 0.0 ? f->opts.incremental_tau : 1e-3 * f->opts.threshold;
 }
 
 // Relative cost of one list: kept primitive quartets times the work of one
@@ -369,6 +389,12 @@ int compute_bounds(hfx_basis* b) {
     dfree(dl.d);
     if (rc) return rc;
     shell_pair_bounds(b->plan, b->bounds.data(), b->Qsh);
+    if (int rc = dalloc(&b->d_qpair, b->Qsh.size())) return rc;
+    if (!b->Qsh.empty()) {
+        HIPCHK(hipMemcpyAsync(b->d_qpair, b->Qsh.data(), b->Qsh.size() * sizeof(double), hipMemcpyHostToDevice,
+                              b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
     b->bounds_ready = true;
     return 0;
 }
@@ -440,6 +466,9 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
     if (!rc) rc = dalloc(&b->d_fscale, (size_t)K);
     if (!rc) rc = dalloc(&b->d_bounds, (size_t)P);
     if (!rc) rc = dalloc(&b->d_fnpairs, (size_t)P);
+    std::vector<int2> shells(p.nshell);
+    for (int i = 0; i < p.nshell; ++i) shells[i] = make_int2(p.shells[i].fn0, ncart(p.shells[i].L));
+    if (!rc) rc = dalloc(&b->d_shells, shells.size());
     hipError_t e = hipSuccess;
     if (!rc) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     for (int i = 0; i < NSIDE; ++i) {
@@ -455,6 +484,7 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
     h2d(b->d_ppf, p.ppf.data(), p.ppf.size() * sizeof(PrimPair<float>));
     h2d(b->d_fscale, p.fscale.data(), K * sizeof(double));
     h2d(b->d_fnpairs, p.fnpairs.data(), P * sizeof(int2));
+    h2d(b->d_shells, shells.data(), shells.size() * sizeof(int2));
     if (!rc && e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (!rc && e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("basis upload: ") + hipGetErrorString(e));
     if (rc) {
@@ -477,6 +507,8 @@ void hfx_basis_destroy(hfx_basis* b) {
     dfree(b->d_fscale);
     dfree(b->d_bounds);
     dfree(b->d_fnpairs);
+    dfree(b->d_shells);
+    dfree(b->d_qpair);
     dfree(b->unique_plan.d);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     for (int i = 0; i < NSIDE; ++i) {
@@ -600,6 +632,8 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     int rc = dalloc(&buf, 6 * K2);
     if (!rc && hipEventCreateWithFlags(&f->ev_done, hipEventDisableTiming) != hipSuccess)
         rc = fail(HFX_ERR_DEVICE, "hipEventCreate");
+    if (!rc && opts->incremental && mode == HFX_MODE_DIRECT)
+        rc = dalloc(&f->d_inc, 8 * K2 + (size_t)b->plan.nshell * b->plan.nshell);
     if (!rc) {
         f->d_Dj = buf;
         f->d_Da = buf + K2;
@@ -631,6 +665,7 @@ void hfx_fock_destroy(hfx_fock* f) {
     if (f->b && f->b->stream) (void)hipStreamSynchronize(f->b->stream);
     if (f->graph) (void)hipGraphExecDestroy(f->graph);
     dfree(f->d_eri);
+    dfree(f->d_inc);
     dfree(f->plan.d);
     dfree(f->d_Dj);   // one allocation for all six matrices
     delete f;
@@ -674,19 +709,15 @@ bool fock_graphs_on() {
 
 // Enqueues one build on `st` (eager, or recorded under stream capture): only
 // kernel launches, no memset / memcpy nodes.
-int enqueue_build(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka, double* d_Kb,
-                  hipStream_t st) {
+// Direct J/K from densities (Dj for J; Da, Db for K) into zeroed, then
+// symmetrised accumulators; dpmax != nullptr screens units by the density
+// difference bound.
+int enqueue_direct(hfx_fock* f, const double* Dj, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka,
+                   double* d_Kb, const double* dpmax, hipStream_t st) {
     hfx_basis* b = f->b;
     const int K = b->plan.K;
     const size_t K2 = (size_t)K * K;
     const int nspin = d_Pb ? 2 : 1;
-    // Dj = Pa (+ Pb) for the Coulomb part
-    const double* Dj = d_Pa;
-    if (nspin == 2) {
-        HIPCHK(launch_add(f->d_Dj, d_Pa, d_Pb, K2, st));
-        Dj = f->d_Dj;
-    }
-    if (f->mode == HFX_MODE_STORED) return jk_from_unique(b, f->d_eri, Dj, d_Pa, d_Pb, d_J, d_Ka, d_Kb, nspin, st);
     HIPCHK(launch_zero(d_J, K2, st));
     HIPCHK(launch_zero(d_Ka, K2, st));
     if (nspin == 2) HIPCHK(launch_zero(d_Kb, K2, st));
@@ -700,10 +731,84 @@ int enqueue_build(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d
     ep.J = d_J;
     ep.Ka = d_Ka;
     ep.Kb = d_Kb;
+    if (dpmax) {
+        ep.dpmax = dpmax;
+        ep.qpair = b->d_qpair;
+        ep.nshell = b->plan.nshell;
+        ep.dthr = incremental_tau(f);
+    }
     if (int rc = launch_plan(b, f->plan, ep, st)) return rc;
     HIPCHK(launch_symmetrize(d_J, K, st));
     HIPCHK(launch_symmetrize(d_Ka, K, st));
     if (nspin == 2) HIPCHK(launch_symmetrize(d_Kb, K, st));
+    return 0;
+}
+
+int enqueue_build(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka, double* d_Kb,
+                  hipStream_t st) {
+    hfx_basis* b = f->b;
+    const size_t K2 = (size_t)b->plan.K * b->plan.K;
+    const int nspin = d_Pb ? 2 : 1;
+    // Dj = Pa (+ Pb) for the Coulomb part
+    const double* Dj = d_Pa;
+    if (nspin == 2) {
+        HIPCHK(launch_add(f->d_Dj, d_Pa, d_Pb, K2, st));
+        Dj = f->d_Dj;
+    }
+    if (f->mode == HFX_MODE_STORED) return jk_from_unique(b, f->d_eri, Dj, d_Pa, d_Pb, d_J, d_Ka, d_Kb, nspin, st);
+    return enqueue_direct(f, Dj, d_Pa, d_Pb, d_J, d_Ka, d_Kb, nullptr, st);
+}
+
+// Incremental direct build: G(P) = G(P_prev) + G(P - P_prev), the second term
+// screened by the shell-block maxima of |P - P_prev|; J / K are linear in the
+// densities and symmetrisation is linear, so the accumulated matrices are the
+// full build's up to the screened terms.  A full build every
+// INCREMENTAL_FULL_EVERY calls (and on the first) bounds how long skipped
+// terms accumulate.
+constexpr int INCREMENTAL_FULL_EVERY = 8;
+
+int enqueue_incremental(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka, double* d_Kb,
+                        hipStream_t st) {
+    hfx_basis* b = f->b;
+    const int K = b->plan.K;
+    const size_t K2 = (size_t)K * K;
+    const int nspin = d_Pb ? 2 : 1;
+    double* Pa0 = f->d_inc;
+    double* Pb0 = Pa0 + K2;
+    double* dPa = Pa0 + 2 * K2;
+    double* dPb = Pa0 + 3 * K2;
+    double* dPj = Pa0 + 4 * K2;
+    double* Jacc = Pa0 + 5 * K2;
+    double* Kaacc = Pa0 + 6 * K2;
+    double* Kbacc = Pa0 + 7 * K2;
+    double* dpm = Pa0 + 8 * K2;
+    if (!f->have_prev || f->prev_nspin != nspin || f->since_full + 1 >= INCREMENTAL_FULL_EVERY) {
+        if (int rc = enqueue_build(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st)) return rc;
+        HIPCHK(launch_copy(Pa0, d_Pa, K2, st));
+        HIPCHK(launch_copy(Jacc, d_J, K2, st));
+        HIPCHK(launch_copy(Kaacc, d_Ka, K2, st));
+        if (nspin == 2) {
+            HIPCHK(launch_copy(Pb0, d_Pb, K2, st));
+            HIPCHK(launch_copy(Kbacc, d_Kb, K2, st));
+        }
+        f->have_prev = true;
+        f->prev_nspin = nspin;
+        f->since_full = 0;
+        return 0;
+    }
+    HIPCHK(launch_delta(dPa, d_Pa, Pa0, K2, st));
+    if (nspin == 2) {
+        HIPCHK(launch_delta(dPb, d_Pb, Pb0, K2, st));
+        HIPCHK(launch_add(dPj, dPa, dPb, K2, st));
+    }
+    HIPCHK(launch_shell_dpmax(b->d_shells, b->plan.nshell, K, dPa, nspin == 2 ? dPb : nullptr,
+                              nspin == 2 ? dPj : nullptr, dpm, st));
+    if (int rc = enqueue_direct(f, nspin == 2 ? dPj : dPa, dPa, nspin == 2 ? dPb : nullptr, d_J, d_Ka, d_Kb, dpm, st))
+        return rc;
+    HIPCHK(launch_accum(d_J, Jacc, K2, st));
+    HIPCHK(launch_accum(d_Ka, Kaacc, K2, st));
+    if (nspin == 2) HIPCHK(launch_accum(d_Kb, Kbacc, K2, st));
+    ++f->since_full;
     return 0;
 }
 
@@ -714,6 +819,11 @@ int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, d
     if (!f || !d_Pa || !d_J || !d_Ka) return fail(HFX_ERR_INVALID, "null argument");
     if (d_Pb && !d_Kb) return fail(HFX_ERR_INVALID, "UHF build needs Kb");
     hipStream_t st = stream ? (hipStream_t)stream : f->b->stream;
+    if (f->d_inc && !g_timer.on) {   // incremental direct builds: eager
+        if (int rc = enqueue_incremental(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st)) return rc;
+        HIPCHK(hipEventRecord(f->ev_done, st));
+        return 0;
+    }
     // class timing records events between launches: always eager; a stored
     // build is a handful of launches: no graph
     if (g_timer.on || f->mode == HFX_MODE_STORED || !fock_graphs_on() || !st) {

@@ -179,6 +179,78 @@ hipError_t launch_add(double* z, const double* x, const double* y, size_t n, hip
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ incremental builds
+// dp = p - prev, then prev = p
+__global__ void k_delta(double* dp, const double* p, double* prev, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = p[i];
+        dp[i] = v - prev[i];
+        prev[i] = v;
+    }
+}
+
+// out += acc, then acc = out
+__global__ void k_accum(double* out, double* acc, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = out[i] + acc[i];
+        out[i] = v;
+        acc[i] = v;
+    }
+}
+
+__global__ void k_copy(double* dst, const double* src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+// out[A * nshell + B] = max over the (A, B) block of |a|, |b|, |c| (b, c may
+// be null); shells[A] = (first function, function count).  One thread per
+// shell pair.
+__global__ void k_shell_dpmax(const int2* shells, int nshell, int K, const double* a, const double* b, const double* c,
+                              double* out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nshell * nshell) return;
+    const int A = (int)(t / nshell), B = (int)(t % nshell);
+    const int2 sa = shells[A], sb = shells[B];
+    double m = 0.0;
+    for (int i = sa.x; i < sa.x + sa.y; ++i)
+        for (int j = sb.x; j < sb.x + sb.y; ++j) {
+            const size_t o = (size_t)i * K + j;
+            m = fmax(m, fabs(a[o]));
+            if (b) m = fmax(m, fabs(b[o]));
+            if (c) m = fmax(m, fabs(c[o]));
+        }
+    out[t] = m;
+}
+
+namespace {
+unsigned grid_1d(size_t n) {
+    size_t blocks = (n + 255) / 256;
+    return (unsigned)(blocks > 65536 ? 65536 : blocks == 0 ? 1 : blocks);
+}
+}  // namespace
+
+hipError_t launch_delta(double* dp, const double* p, double* prev, size_t n, hipStream_t st) {
+    k_delta<<<grid_1d(n), 256, 0, st>>>(dp, p, prev, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_accum(double* out, double* acc, size_t n, hipStream_t st) {
+    k_accum<<<grid_1d(n), 256, 0, st>>>(out, acc, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st) {
+    k_copy<<<grid_1d(n), 256, 0, st>>>(dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_shell_dpmax(const int2* shells, int nshell, int K, const double* a, const double* b, const double* c,
+                              double* out, hipStream_t st) {
+    const int64_t n = (int64_t)nshell * nshell;
+    k_shell_dpmax<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(shells, nshell, K, a, b, c, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero(double* p, size_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;

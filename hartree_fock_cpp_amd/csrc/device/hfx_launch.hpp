@@ -27,5 +27,11 @@ hipError_t launch_symmetrize(double* M, int K, hipStream_t st);
 hipError_t launch_zero(double* p, size_t n, hipStream_t st);
 hipError_t launch_axpy(double* y, const double* x, size_t n, hipStream_t st);   // y += x
 hipError_t launch_add(double* z, const double* x, const double* y, size_t n, hipStream_t st);   // z = x + y
+// incremental J/K builds
+hipError_t launch_delta(double* dp, const double* p, double* prev, size_t n, hipStream_t st);   // dp = p - prev; prev = p
+hipError_t launch_accum(double* out, double* acc, size_t n, hipStream_t st);                    // out += acc; acc = out
+hipError_t launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
+hipError_t launch_shell_dpmax(const int2* shells, int nshell, int K, const double* a, const double* b, const double* c,
+                              double* out, hipStream_t st);
 
 }  // namespace hfx

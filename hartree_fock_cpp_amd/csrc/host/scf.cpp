@@ -40,6 +40,10 @@ struct Engine {
         fo.threshold = o.screening_threshold;
         fo.mixed_split = o.mixed_split;
         fo.hbm_budget = o.hbm_budget;
+        fo.incremental = o.incremental_fock != 0;
+        // skipped difference terms three orders below both the integral screen
+        // and the convergence threshold, so they cannot stall convergence
+        fo.incremental_tau = 1e-3 * std::min(o.screening_threshold, o.epsilon);
         check(hfx_fock_create(b, &fo, &f));
     }
 };

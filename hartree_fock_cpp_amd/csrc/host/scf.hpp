@@ -37,6 +37,7 @@ struct ScfOptions {
     bool polish = true;
     bool verbose = true;
     int device_iteration = -1;     // 1 GPU-resident iteration, 0 host, -1 auto (K >= 96)
+    int incremental_fock = -1;     // direct builds from density differences: 1 on, 0 off, -1 auto (on)
 };
 
 struct ScfTimings {

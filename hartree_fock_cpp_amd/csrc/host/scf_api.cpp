@@ -41,6 +41,7 @@ hfxh::ScfOptions to_opts(const hfx_scf_opts* o) {
     r.polish = o->polish != 0;
     r.verbose = o->verbose != 0;
     r.device_iteration = o->device_iteration;
+    r.incremental_fock = o->incremental_fock;
     return r;
 }
 
@@ -86,6 +87,7 @@ void hfx_scf_default_opts(hfx_scf_opts* o) {
     o->polish = 1;
     o->verbose = 0;
     o->device_iteration = -1;
+    o->incremental_fock = -1;
 }
 
 int hfx_basis_size(int natom, const char* symbols, const char* basis_set, const char* basis_path, int* K) {

@@ -119,6 +119,7 @@ struct hfx_server {
         o.polish = j->scf.polish != 0;
         o.verbose = false;
         o.device_iteration = j->scf.device_iteration;
+        o.incremental_fock = j->scf.incremental_fock;
         try {
             auto fill = [&](auto& r, int Ne) {
                 hfx_scf_result& s = j->res.scf;

@@ -87,7 +87,8 @@ def ERI_GPU(basis, threshold):
 class FockBuilder:
     """One SCF job's Fock-build context (stored or direct, FP64 or mixed)."""
 
-    def __init__(self, basis, mode="auto", precision="fp64", threshold=1e-8, mixed_split=1e-5, hbm_budget=0):
+    def __init__(self, basis, mode="auto", precision="fp64", threshold=1e-8, mixed_split=1e-5, hbm_budget=0,
+                 incremental=False):
         self.basis = basis
         o = N.FockOpts()
         o.mode = {"stored": N.HFX_MODE_STORED, "direct": N.HFX_MODE_DIRECT, "auto": N.HFX_MODE_AUTO}[mode]
@@ -95,6 +96,7 @@ class FockBuilder:
         o.threshold = threshold
         o.mixed_split = mixed_split
         o.hbm_budget = hbm_budget
+        o.incremental = int(bool(incremental))
         self.opts = o
         h = C.c_void_p()
         N.check(N.lib.hfx_fock_create(basis.handle(), C.byref(o), C.byref(h)))

@@ -23,7 +23,7 @@ class ScfOpts(C.Structure):
                 ("epsilon", C.c_double), ("damping", C.c_double), ("screening_threshold", C.c_double),
                 ("eri_mode", C.c_int), ("precision", C.c_int), ("mixed_split", C.c_double),
                 ("energy_convergence", C.c_double), ("hbm_budget", C.c_size_t), ("polish", C.c_int),
-                ("verbose", C.c_int), ("device_iteration", C.c_int)]
+                ("verbose", C.c_int), ("device_iteration", C.c_int), ("incremental_fock", C.c_int)]
 
 
 class ScfResult(C.Structure):
@@ -67,7 +67,7 @@ DEVICE_ITERATION = {"auto": -1, "on": 1, "off": 0, True: 1, False: 0, -1: -1, 1:
 
 
 def _opts(charge, max_iterations, epsilon, damping, screening_threshold, eri_mode, precision, mixed_split,
-          energy_convergence, hbm_budget, polish, verbose, spin=None, device_iteration="auto"):
+          energy_convergence, hbm_budget, polish, verbose, spin=None, device_iteration="auto", incremental_fock="auto"):
     o = ScfOpts()
     N.lib.hfx_scf_default_opts(C.byref(o))
     o.charge = charge
@@ -83,6 +83,7 @@ def _opts(charge, max_iterations, epsilon, damping, screening_threshold, eri_mod
     o.polish = int(polish)
     o.verbose = int(verbose)
     o.device_iteration = DEVICE_ITERATION[device_iteration]
+    o.incremental_fock = DEVICE_ITERATION[incremental_fock]
     if spin is not None:
         o.n_alpha, o.n_beta = spin
     return o
@@ -90,12 +91,15 @@ def _opts(charge, max_iterations, epsilon, damping, screening_threshold, eri_mod
 
 def rhf_diis(config, basis_set, basis_path=BASIS_DIR, charge=0, P0=None, max_iterations=10000, epsilon=1e-6,
              damping=0.3, screening_threshold=1e-8, eri_mode="auto", precision="fp64", mixed_split=1e-5,
-             energy_convergence=0.0, hbm_budget=0, polish=True, verbose=False, device_iteration="auto"):
+             energy_convergence=0.0, hbm_budget=0, polish=True, verbose=False, device_iteration="auto",
+             incremental_fock="auto"):
     """Returns a dict with energies, timings, density, coefficient, mo_energy.
-    device_iteration: "auto" (GPU-resident iteration for K >= 96), "on" or "off"."""
+    device_iteration: "auto" (GPU-resident iteration for K >= 96), "on" or "off".
+    incremental_fock: direct builds from density differences, "auto" (on), "on" or "off"."""
     K = basis_size(config, basis_set, basis_path)
     o = _opts(charge, max_iterations, epsilon, damping, screening_threshold, eri_mode, precision, mixed_split,
-              energy_convergence, hbm_budget, polish, verbose, device_iteration=device_iteration)
+              energy_convergence, hbm_budget, polish, verbose, device_iteration=device_iteration,
+              incremental_fock=incremental_fock)
     res = ScfResult()
     P, Cm, e = np.zeros((K, K)), np.zeros((K, K)), np.zeros(K)
     p0 = N.as_f64(P0) if P0 is not None else None
@@ -110,10 +114,12 @@ def rhf_diis(config, basis_set, basis_path=BASIS_DIR, charge=0, P0=None, max_ite
 
 def uhf_diis(config, basis_set, spin_occ, basis_path=BASIS_DIR, charge=0, max_iterations=10000, epsilon=1e-6,
              damping=0.3, screening_threshold=1e-8, eri_mode="auto", precision="fp64", mixed_split=1e-5,
-             energy_convergence=0.0, hbm_budget=0, polish=True, verbose=False, device_iteration="auto"):
+             energy_convergence=0.0, hbm_budget=0, polish=True, verbose=False, device_iteration="auto",
+             incremental_fock="auto"):
     K = basis_size(config, basis_set, basis_path)
     o = _opts(charge, max_iterations, epsilon, damping, screening_threshold, eri_mode, precision, mixed_split,
-              energy_convergence, hbm_budget, polish, verbose, spin=spin_occ, device_iteration=device_iteration)
+              energy_convergence, hbm_budget, polish, verbose, spin=spin_occ, device_iteration=device_iteration,
+              incremental_fock=incremental_fock)
     res = ScfResult()
     Pa, Pb, Ca, Cb = (np.zeros((K, K)) for _ in range(4))
     ea, eb = np.zeros(K), np.zeros(K)

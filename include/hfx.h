@@ -105,6 +105,11 @@ typedef struct {
     double threshold;      /* function-level Schwarz threshold (reference screening_threshold) */
     double mixed_split;    /* quartet bound below which the mixed form uses FP32 (default 1e-5) */
     size_t hbm_budget;     /* bytes this job may hold on the device; 0 = unlimited */
+    int incremental;       /* direct mode: 1 = build G(P) as G(P_prev) + G(P - P_prev) with the second
+                              term screened by |P - P_prev| (full build every 8th call); 0 = off.
+                              No reference counterpart (the reference rebuilds in full every iteration). */
+    double incremental_tau;/* skip a quartet of the difference term when Q_ab Q_cd max|dP| < tau;
+                              0 = 1e-3 x threshold */
 } hfx_fock_opts;
 
 /* Fock-build context for one SCF job: owns its stored ERIs (stored mode) or

@@ -38,6 +38,8 @@ typedef struct {
     int verbose;
     int device_iteration;      /* 1: SCF iteration resident on the GPU (Fock assembly, DIIS, transforms,
                                   densities; host eigen-solve), 0: on the host, -1: auto (K >= 96) */
+    int incremental_fock;      /* direct builds from density differences (hfx_fock_opts.incremental):
+                                  1 on, 0 off, -1 auto (on) */
 } hfx_scf_opts;
 
 typedef struct {

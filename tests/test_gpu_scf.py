@@ -104,6 +104,22 @@ def test_mixed_precision_with_polish(gpu):
     assert np.abs(rmx["density"] - r64["density"]).max() < 1e-6
 
 
+def test_incremental_fock_scf_matches_full(gpu):
+    """Direct SCF with incremental Fock builds converges to the energy and
+    density of the full-build SCF (host and GPU-resident iterations)."""
+    from hartree_fock_cpp_amd.scf import rhf_diis
+    d = golden("h2o_ccpvdz.npz")
+    cfg = _config(d)
+    for dev in ("off", "on"):
+        r0 = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-9, eri_mode="direct", incremental_fock="off", device_iteration=dev)
+        r1 = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-9, eri_mode="direct", incremental_fock="on", device_iteration=dev)
+        print(f"device_iteration={dev}: full E={r0['E_tot']:.12f} ({r0['iterations']} it), "
+              f"incremental E={r1['E_tot']:.12f} ({r1['iterations']} it)")
+        assert r0["converged"] and r1["converged"]
+        assert abs(r0["E_tot"] - r1["E_tot"]) < 1e-10
+        assert np.abs(r0["density"] - r1["density"]).max() < 1e-7
+
+
 def test_benzene_ccpvdz_full_size_properties(gpu):
     """Benzene RHF/cc-pVDZ at full size: stored and direct agree, the density
     is idempotent (P S P = 2 P) and holds 42 electrons."""
