@@ -487,7 +487,9 @@ struct HermTab {
     unsigned char t[455], u[455], v[455];
 };
 
-template <int LA, int LB, int LC, int LD, class Real>
+// JK = false (write epilogues) stages only the scales and bounds, so those
+// kernels hold a smaller LDS block (more resident wavefronts per CU).
+template <int LA, int LB, int LC, int LD, class Real, bool JK = true>
 struct FamBSmem {
     using S = FamBShape<LA, LB, LC, LD>;
     Real R[S::KB][2][S::NRP];          // packed ping-pong levels (not S::DENSE)
@@ -501,7 +503,7 @@ struct FamBSmem {
     unsigned char ht[S::NR][3];        // packed Hermite index -> (t, u, v)
     unsigned char abc[S::NAB][6];      // bra component pair -> (ax, ay, az, bx, by, bz)
     unsigned char cdc[S::NCD][6];      // ket component pair -> (cx, cy, cz, dx, dy, dz)
-    double epi[S::NEPI];               // epilogue staging, per quartet (offsets in FamBShape)
+    double epi[JK ? S::NEPI : S::O_DJAB];   // epilogue staging, per quartet (offsets in FamBShape)
     Real R3[S::KB][S::ND3];            // dense R cube (S::DENSE)
     // R recursion sources per packed Hermite index w (class constants):
     // packed levels: idx1 | idx2 << 10 | coef << 20 | axis << 26 (axis 3: the base a[n]);
@@ -516,7 +518,7 @@ struct FamBSmem {
 // (not for EPI_DIAG, which is producing them) and, for J/K, the density
 // blocks, so the epilogue reads LDS instead of HBM / L2.
 template <int LA, int LB, int LC, int LD, class Real, bool JK>
-HFX_HD __forceinline__ void famb_stage_epi(FamBSmem<LA, LB, LC, LD, Real>& sm, const QuartetCtx& q,
+HFX_HD __forceinline__ void famb_stage_epi(FamBSmem<LA, LB, LC, LD, Real, JK>& sm, const QuartetCtx& q,
                                            const EpiArgs& ep, int lane, int nlanes) {
     using S = FamBShape<LA, LB, LC, LD>;
     double* E = sm.epi;
@@ -563,8 +565,8 @@ HFX_HD __forceinline__ void famb_stage_epi(FamBSmem<LA, LB, LC, LD, Real>& sm, c
 }
 
 // Fills the decode tables of a family-B workgroup (lanes stride the entries).
-template <int LA, int LB, int LC, int LD, class Real>
-HFX_HD __forceinline__ void famb_tables(FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht, int lane, int nlanes) {
+template <int LA, int LB, int LC, int LD, class Real, class SM>
+HFX_HD __forceinline__ void famb_tables(SM& sm, const HermTab& ht, int lane, int nlanes) {
     using S = FamBShape<LA, LB, LC, LD>;
     for (int w = lane; w < S::NR; w += nlanes) {
         sm.ht[w][0] = ht.t[w];
@@ -637,7 +639,7 @@ HFX_HD __forceinline__ void build_e_lanes(Real* dst, const PrimPair<Real>& pp, b
 // `sync()` must be a wavefront/workgroup barrier on the device.
 template <int LA, int LB, int LC, int LD, class Real, bool JK, class Sync>
 HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<Real>* pp,
-                         const Real* tab, const EpiArgs& ep, FamBSmem<LA, LB, LC, LD, Real>& sm, const HermTab& ht,
+                         const Real* tab, const EpiArgs& ep, FamBSmem<LA, LB, LC, LD, Real, JK>& sm, const HermTab& ht,
                          int lane, int nlanes, Sync sync) {
     using S = FamBShape<LA, LB, LC, LD>;
     constexpr int SB = (LA + 1) * (LB + 1) * (S::LAB + 1);

@@ -32,7 +32,7 @@ template <int LA, int LB, int LC, int LD, class Real, bool JK>
 __global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
                                                const HermTab* __restrict__ ht, EpiArgs ep) {
-    __shared__ FamBSmem<LA, LB, LC, LD, Real> sm;
+    __shared__ FamBSmem<LA, LB, LC, LD, Real, JK> sm;
     auto sync = []() { __syncthreads(); };
     constexpr int NCH = FamBShape<LA, LB, LC, LD>::NCHUNK;
     const long long total = (long long)n * NCH;

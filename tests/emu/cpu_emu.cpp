@@ -79,7 +79,8 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucke
         tab = c.boysf.data();
     }
     size_t pos = 0;
-    auto sm = fam_a ? nullptr : std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real>>();
+    auto smj = fam_a ? nullptr : std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real, true>>();
+    auto smw = fam_a ? nullptr : std::make_unique<FamBSmem<C::LA, C::LB, C::LC, C::LD, Real, false>>();
     for (int g = 0; g < NGROUP; ++g) {
         const std::vector<uint2> units = emu_units(c, fam_a, GROUP_SIZE[g], list.data() + pos, bucket[g], ep,
                                                    famb_kb(C::LA + C::LB + C::LC + C::LD));
@@ -94,10 +95,10 @@ void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucke
                 for (int ch = 0; ch < NCH; ++ch) {   // one device wavefront per (unit, chunk)
                     if (ep.mode == EPI_JK)
                         family_b_one<C::LA, C::LB, C::LC, C::LD, Real, true>(e, ch, c.plan.hdr.data(), pp, tab, ep,
-                                                                             *sm, c.ht, 0, 1, sync);
+                                                                             *smj, c.ht, 0, 1, sync);
                     else
                         family_b_one<C::LA, C::LB, C::LC, C::LD, Real, false>(e, ch, c.plan.hdr.data(), pp, tab, ep,
-                                                                              *sm, c.ht, 0, 1, sync);
+                                                                              *smw, c.ht, 0, 1, sync);
                 }
             }
         }
