@@ -27,9 +27,16 @@ __global__ __launch_bounds__(FAM_A_BLOCK) void k_fam_a(const uint2* __restrict__
                                                 tab, ep);
 }
 
+// Minimum resident wavefronts per SIMD the register allocator must leave room
+// for: without the hint it takes up to 128-165 VGPRs (3-4 waves) where a few
+// bytes of spill buy a fifth.  Measured per class on the benzene unique build
+// and the (H2O)4 / n-octane direct builds: 5 for the write kernels, 4 for J/K.
+template <bool JK>
+constexpr int famb_waves_per_eu() { return JK ? 4 : 5; }
+
 // One wavefront per (work unit, ket-component chunk), grid-strided.
 template <int LA, int LB, int LC, int LD, class Real, bool JK>
-__global__ __launch_bounds__(64) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(famb_waves_per_eu<JK>()))) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
                                                const HermTab* __restrict__ ht, EpiArgs ep) {
     __shared__ FamBSmem<LA, LB, LC, LD, Real, JK> sm;
