@@ -48,7 +48,10 @@ HFX_HD __forceinline__ void static_for(F&& f) {
 }
 
 // Family A is used while the output block, R_tuv and the E tables stay within
-// about 96 FP64 registers' worth of values.
+// about 96 FP64 registers' worth of values.  Measured: moving the classes with
+// 64 < nout + nherm(L) <= 128 ((pp|pp), (ds|pp), (dp|ps), ...) to family B
+// slows the benzene unique build 1.37x although family A runs them at one or
+// two resident waves per SIMD.
 __host__ __device__ constexpr bool use_family_a(int la, int lb, int lc, int ld) {
     int L = la + lb + lc + ld;
     int nout = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
