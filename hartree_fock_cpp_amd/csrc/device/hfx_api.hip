@@ -147,6 +147,7 @@ namespace {
 // Optional per-class timing of ERI launches (HIP events on the launch stream).
 struct ClassTimer {
     bool on = false;
+    bool serial = false;   // every list on the caller's stream: each class has the device to itself
     hipEvent_t ev[NQCLASS][2][2] = {};
     bool used[NQCLASS][2] = {};
     bool created = false;
@@ -219,13 +220,14 @@ double list_cost(const DevLists& dl, int qc, int p) {
 int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipStream_t st) {
     // Lists go, largest first, to the least loaded of the caller's stream and
     // the basis's side streams; the caller's stream then waits for the others.
-    // Class timing keeps every launch on the caller's stream.
+    // Class timing brackets each list with events on the stream it runs on
+    // (serial timing: all lists on the caller's stream).
     static const int nlaunch = [] {
         const char* e = std::getenv("HFX_LAUNCH_STREAMS");
         const int v = e ? std::atoi(e) : NSIDE + 1;
         return v < 1 ? 1 : v > NSIDE + 1 ? NSIDE + 1 : v;
     }();
-    const bool fan = !g_timer.on && b->ev_fork && nlaunch > 1;
+    const bool fan = b->ev_fork && nlaunch > 1 && !(g_timer.on && g_timer.serial);
     hipStream_t lane_st[NSIDE + 1] = {st};
     for (int i = 0; i < NSIDE; ++i) lane_st[i + 1] = b->side[i];
     int order[NQCLASS * 2], nl = 0;
@@ -253,7 +255,7 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
         }
         hipStream_t ls = lane_st[lane];
         {
-            if (g_timer.on) HIPCHK(hipEventRecord(g_timer.ev[qc][p][0], st));
+            if (g_timer.on) HIPCHK(hipEventRecord(g_timer.ev[qc][p][0], ls));
             size_t off = dl.off[qc][p];
             for (int g = 0; g < NGROUP; ++g) {
                 const int n = dl.bucket[qc][p][g];
@@ -271,7 +273,7 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
                 off += n;
             }
             if (g_timer.on) {
-                HIPCHK(hipEventRecord(g_timer.ev[qc][p][1], st));
+                HIPCHK(hipEventRecord(g_timer.ev[qc][p][1], ls));
                 g_timer.used[qc][p] = true;
             }
         }
@@ -885,6 +887,7 @@ int hfx_set_class_timing(int on) {
         g_timer.created = true;
     }
     g_timer.on = on != 0;
+    g_timer.serial = on == 2;
     return 0;
 }
 

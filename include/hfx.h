@@ -143,7 +143,10 @@ int hfx_fock_class_counts(const hfx_fock* f, long long* n_fp64, long long* n_fp3
 /* Measurement hooks (no reference counterpart).  With class timing on, every
  * ERI class launch is bracketed by HIP events on its stream; hfx_class_times
  * returns the last build's per-launch milliseconds, ms[2*qc + p] with p = 0
- * for FP64 and 1 for FP32 (0 for classes not launched). */
+ * for FP64 and 1 for FP32 (0 for classes not launched).  on = 1: the lists
+ * of one build still run on up to four streams at once, so a class's time
+ * includes the device it shares with the classes beside it; on = 2: every
+ * list runs on the caller's stream, one class at a time. */
 int hfx_set_class_timing(int on);
 
 /* Graph replay of the Fock build (no reference counterpart): each hfx_fock
