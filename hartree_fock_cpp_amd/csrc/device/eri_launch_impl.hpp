@@ -28,15 +28,27 @@ __global__ __launch_bounds__(FAM_A_BLOCK) void k_fam_a(const uint2* __restrict__
 }
 
 // Minimum resident wavefronts per SIMD the register allocator must leave room
-// for: without the hint it takes up to 128-165 VGPRs (3-4 waves) where a few
-// bytes of spill buy a fifth.  Measured per class on the benzene unique build
-// and the (H2O)4 / n-octane direct builds: 5 for the write kernels, 4 for J/K.
-template <bool JK>
-constexpr int famb_waves_per_eu() { return JK ? 4 : 5; }
+// for (1: no constraint).  Unconstrained, family-B kernels take 128-165 VGPRs
+// (3-4 waves); for the classes below a few bytes of spill buy a fourth or
+// fifth wave and pay, elsewhere the spill traffic costs more than the
+// occupancy gains.  Chosen per class from a sweep of 1 / 4 / 5 / 6 on the
+// benzene / cc-pVDZ unique build (write) and the (H2O)4 / cc-pVTZ and
+// n-octane / cc-pVDZ direct builds (J/K); a class keeps 1 unless it gains 2 %.
+template <int LA, int LB, int LC, int LD, bool JK>
+constexpr int famb_waves_per_eu() {
+    constexpr int c = ((LA * 4 + LB) * 4 + LC) * 4 + LD;
+    constexpr auto id = [](int a, int b, int cc, int d) { return ((a * 4 + b) * 4 + cc) * 4 + d; };
+    if constexpr (!JK) return (c == id(2, 1, 2, 0) || c == id(2, 2, 1, 0)) ? 5 : 1;
+    if (c == id(2, 2, 1, 0) || c == id(3, 0, 2, 0)) return 5;
+    if (c == id(2, 1, 1, 1) || c == id(2, 1, 2, 0) || c == id(2, 2, 1, 1) || c == id(3, 0, 3, 0) ||
+        c == id(3, 1, 1, 0) || c == id(3, 1, 3, 0) || c == id(3, 2, 0, 0) || c == id(3, 2, 1, 0) || c == id(3, 3, 1, 0))
+        return 4;
+    return 1;
+}
 
 // One wavefront per (work unit, ket-component chunk), grid-strided.
 template <int LA, int LB, int LC, int LD, class Real, bool JK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(famb_waves_per_eu<JK>()))) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(famb_waves_per_eu<LA, LB, LC, LD, JK>()))) void k_fam_b(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
                                                const PrimPair<Real>* __restrict__ pp, const Real* __restrict__ tab,
                                                const HermTab* __restrict__ ht, EpiArgs ep) {
     __shared__ FamBSmem<LA, LB, LC, LD, Real, JK> sm;
