@@ -24,13 +24,15 @@ void gemm(int n, const double* A, bool ta, const double* B, bool tb, double* C) 
             for (int j = 0; j < n; ++j) Bt[(size_t)i * n + j] = B[(size_t)j * n + i];
         b = Bt.data();
     }
+    // C must not overlap A or B (callers pass distinct matrices): the inner
+    // loop then vectorises
     std::fill(C, C + (size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) {
-        double* c = C + (size_t)i * n;
+        double* __restrict__ c = C + (size_t)i * n;
         for (int k = 0; k < n; ++k) {
             const double aik = a[(size_t)i * n + k];
             if (aik == 0.0) continue;
-            const double* bk = b + (size_t)k * n;
+            const double* __restrict__ bk = b + (size_t)k * n;
             for (int j = 0; j < n; ++j) c[j] += aik * bk[j];
         }
     }
@@ -48,7 +50,7 @@ void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::
     e.assign(n, 0.0);
     Qc.assign((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) Qc[(size_t)i * n + i] = 1.0;
-    std::vector<double> v(n), p(n), w(n);
+    std::vector<double> v(n), p(n), w(n), s(n);
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;   // length of x = A[k+1.., k]
         double xn = 0.0;
@@ -64,22 +66,32 @@ void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::
         for (int i = 0; i < m; ++i) vv += v[i] * v[i];
         if (vv == 0.0) continue;
         const double beta = 2.0 / vv;
-        // p = beta * A' v  (A' = trailing block)
+        // p = beta * A' v  (A' = trailing block; four partial sums so the
+        // dot products vectorise)
+        const double* __restrict__ vp = v.data();
         for (int i = 0; i < m; ++i) {
-            const double* row = &A[(size_t)(k + 1 + i) * n + k + 1];
-            double s = 0.0;
-            for (int j = 0; j < m; ++j) s += row[j] * v[j];
-            p[i] = beta * s;
+            const double* __restrict__ row = &A[(size_t)(k + 1 + i) * n + k + 1];
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int j = 0;
+            for (; j + 4 <= m; j += 4) {
+                s0 += row[j] * vp[j];
+                s1 += row[j + 1] * vp[j + 1];
+                s2 += row[j + 2] * vp[j + 2];
+                s3 += row[j + 3] * vp[j + 3];
+            }
+            for (; j < m; ++j) s0 += row[j] * vp[j];
+            p[i] = beta * ((s0 + s1) + (s2 + s3));
         }
         double pv = 0.0;
         for (int i = 0; i < m; ++i) pv += p[i] * v[i];
         const double c = 0.5 * beta * pv;
         for (int i = 0; i < m; ++i) w[i] = p[i] - c * v[i];
         // A' -= v w^T + w v^T
+        const double* __restrict__ wp = w.data();
         for (int i = 0; i < m; ++i) {
-            double* row = &A[(size_t)(k + 1 + i) * n + k + 1];
-            const double vi = v[i], wi = w[i];
-            for (int j = 0; j < m; ++j) row[j] -= vi * w[j] + wi * v[j];
+            double* __restrict__ row = &A[(size_t)(k + 1 + i) * n + k + 1];
+            const double vi = vp[i], wi = wp[i];
+            for (int j = 0; j < m; ++j) row[j] -= vi * wp[j] + wi * vp[j];
         }
         A[(size_t)(k + 1) * n + k] = alpha;
         A[(size_t)k * n + k + 1] = alpha;
@@ -87,13 +99,19 @@ void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::
             A[(size_t)(k + 1 + i) * n + k] = 0.0;
             A[(size_t)k * n + k + 1 + i] = 0.0;
         }
-        // Q <- Q H, H = I - beta v v^T acting on columns k+1..n-1
-        for (int r = 0; r < n; ++r) {
-            double s = 0.0;
-            for (int j = 0; j < m; ++j) s += Qc[(size_t)(k + 1 + j) * n + r] * v[j];
-            s *= beta;
-            if (s == 0.0) continue;
-            for (int j = 0; j < m; ++j) Qc[(size_t)(k + 1 + j) * n + r] -= s * v[j];
+        // Q <- Q H, H = I - beta v v^T acting on columns k+1..n-1: s = Q v, then
+        // Q -= beta s v^T, both passes streaming whole columns of Qc
+        std::fill(s.begin(), s.end(), 0.0);
+        double* __restrict__ sp = s.data();
+        for (int j = 0; j < m; ++j) {
+            const double vj = vp[j];
+            const double* __restrict__ col = &Qc[(size_t)(k + 1 + j) * n];
+            for (int r = 0; r < n; ++r) sp[r] += col[r] * vj;
+        }
+        for (int j = 0; j < m; ++j) {
+            const double bv = beta * vp[j];
+            double* __restrict__ col = &Qc[(size_t)(k + 1 + j) * n];
+            for (int r = 0; r < n; ++r) col[r] -= sp[r] * bv;
         }
     }
     for (int i = 0; i < n; ++i) d[i] = A[(size_t)i * n + i];
@@ -150,9 +168,9 @@ void tridiag_qr(int n, std::vector<double>& d, std::vector<double>& e, std::vect
                 z = -sn * e[k + 1];
                 e[k + 1] = cs * e[k + 1];
             }
-            // Q <- Q G
-            double* qk = &Qc[(size_t)k * n];
-            double* qk1 = &Qc[(size_t)(k + 1) * n];
+            // Q <- Q G (distinct columns: no aliasing, the loop vectorises)
+            double* __restrict__ qk = &Qc[(size_t)k * n];
+            double* __restrict__ qk1 = &Qc[(size_t)(k + 1) * n];
             for (int r = 0; r < n; ++r) {
                 const double u = qk[r], w = qk1[r];
                 qk[r] = cs * u - sn * w;
