@@ -174,6 +174,9 @@ struct hfx_fock {
     // incremental direct builds: previous densities, density differences,
     // accumulated (symmetrised) J / K, shell-block maxima of |dP|
     double* d_inc = nullptr;
+    // pinned staging of the host-matrix entry point (Pa, Pb in; J, Ka, Kb out)
+    double* h_stage = nullptr;
+    double* h_stage_dev = nullptr;
     bool have_prev = false;
     int prev_nspin = 0;
     int since_full = 0;
@@ -668,6 +671,7 @@ void hfx_fock_destroy(hfx_fock* f) {
     if (f->graph) (void)hipGraphExecDestroy(f->graph);
     dfree(f->d_eri);
     dfree(f->d_inc);
+    if (f->h_stage) hfx_detail::host_free(f->h_stage);
     dfree(f->plan.d);
     dfree(f->d_Dj);   // one allocation for all six matrices
     delete f;
@@ -863,19 +867,44 @@ int hfx_set_fock_graphs(int on) {
     return 0;
 }
 
+// Host matrices travel through a pinned staging block of the context: the
+// densities are copied into it on the CPU and read by a copy kernel over the
+// host link, the results written back the same way.  Copies of the caller's
+// pageable memory would each take an SDMA round trip (measured ~0.1 ms apiece
+// from about 10 KB on, the bulk of a small molecule's build).
 int hfx_fock_build(hfx_fock* f, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb) {
     if (!f || !Pa) return fail(HFX_ERR_INVALID, "null argument");
     hfx_basis* b = f->b;
     const int K = b->plan.K;
     const size_t K2 = (size_t)K * K;
-    HIPCHK(hipMemcpyAsync(f->d_Da, Pa, K2 * sizeof(double), hipMemcpyHostToDevice, b->stream));
-    if (Pb) HIPCHK(hipMemcpyAsync(f->d_Db, Pb, K2 * sizeof(double), hipMemcpyHostToDevice, b->stream));
-    if (int rc = hfx_fock_build_device(f, f->d_Da, Pb ? f->d_Db : nullptr, f->d_J, f->d_Ka, f->d_Kb, b->stream))
-        return rc;
-    if (J) HIPCHK(hipMemcpyAsync(J, f->d_J, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-    if (Ka) HIPCHK(hipMemcpyAsync(Ka, f->d_Ka, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-    if (Kb && Pb) HIPCHK(hipMemcpyAsync(Kb, f->d_Kb, K2 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
+    hipStream_t st = b->stream;
+    if (!f->h_stage) {
+        void* h = nullptr;
+        HIPCHK(hfx_detail::host_alloc(&h, 5 * K2 * sizeof(double)));
+        f->h_stage = static_cast<double*>(h);
+        void* dv = nullptr;
+        const hipError_t e = hipHostGetDevicePointer(&dv, h, 0);
+        if (e != hipSuccess) {
+            hfx_detail::host_free(h);
+            f->h_stage = nullptr;
+            HIPCHK(e);
+        }
+        f->h_stage_dev = static_cast<double*>(dv);
+    }
+    double* h = f->h_stage;
+    double* hd = f->h_stage_dev;   // the same block, as the device addresses it
+    std::memcpy(h, Pa, K2 * sizeof(double));
+    if (Pb) std::memcpy(h + K2, Pb, K2 * sizeof(double));
+    HIPCHK(launch_copy(f->d_Da, hd, K2, st));
+    if (Pb) HIPCHK(launch_copy(f->d_Db, hd + K2, K2, st));
+    if (int rc = hfx_fock_build_device(f, f->d_Da, Pb ? f->d_Db : nullptr, f->d_J, f->d_Ka, f->d_Kb, st)) return rc;
+    if (J) HIPCHK(launch_copy(hd + 2 * K2, f->d_J, K2, st));
+    if (Ka) HIPCHK(launch_copy(hd + 3 * K2, f->d_Ka, K2, st));
+    if (Kb && Pb) HIPCHK(launch_copy(hd + 4 * K2, f->d_Kb, K2, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (J) std::memcpy(J, h + 2 * K2, K2 * sizeof(double));
+    if (Ka) std::memcpy(Ka, h + 3 * K2, K2 * sizeof(double));
+    if (Kb && Pb) std::memcpy(Kb, h + 4 * K2, K2 * sizeof(double));
     return 0;
 }
 

@@ -196,6 +196,13 @@ __global__ void k_damp(size_t n, double d, const double* __restrict__ Pn, double
         P[x] = d * Pn[x] + (1.0 - d) * P[x];
 }
 
+// Copies between device memory and pinned host memory go through this kernel
+// (the GPU reads / writes the pinned block over the host link): an SDMA copy
+// of the same few kilobytes costs ~0.1 ms of latency.
+__global__ void k_copy(size_t n, const double* __restrict__ src, double* __restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 __global__ void k_fill(size_t n, double v, double* p) {
     for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (size_t)gridDim.x * blockDim.x) p[x] = v;
 }
@@ -224,6 +231,12 @@ struct DevScf::Impl {
     double *T1 = nullptr, *T2 = nullptr;                       // K2
     double *sc = nullptr, *coef = nullptr;                     // SC_N, MAXH
     double *h_sc = nullptr, *h_mat = nullptr, *h_coef = nullptr;   // pinned host
+    double *h_sc_dev = nullptr, *h_mat_dev = nullptr, *h_coef_dev = nullptr;   // the same, as the device sees them
+
+    void copy(const double* src, double* dst, size_t n) {
+        k_copy<<<grid_for(n), 256, 0, st>>>(n, src, dst);
+        hchk(hipGetLastError(), "copy launch");
+    }
 
     void gemm(int M, int N, int Kd, double alpha, const double* A, int lda, bool ta, const double* B, int ldb,
               bool tb, double* Cm, int ldc) {
@@ -233,7 +246,7 @@ struct DevScf::Impl {
     }
     void zero_scalars() { k_fill<<<1, 64, 0, st>>>(SC_N, 0.0, sc); }
     void read_scalars() {
-        hchk(hipMemcpyAsync(h_sc, sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, st), "scalar read-back");
+        copy(sc, h_sc_dev, SC_N);
         hchk(hipStreamSynchronize(st), "sync");
     }
 
@@ -291,6 +304,9 @@ DevScf::DevScf(int K, int nspin, const int* nocc, const double* H, const double*
         hchk(hfx_detail::host_alloc((void**)&d.h_sc, SC_N * sizeof(double)), "pinned allocation");
         hchk(hfx_detail::host_alloc((void**)&d.h_mat, K2 * sizeof(double)), "pinned allocation");
         hchk(hfx_detail::host_alloc((void**)&d.h_coef, MAXH * sizeof(double)), "pinned allocation");
+        hchk(hipHostGetDevicePointer((void**)&d.h_sc_dev, d.h_sc, 0), "pinned device pointer");
+        hchk(hipHostGetDevicePointer((void**)&d.h_mat_dev, d.h_mat, 0), "pinned device pointer");
+        hchk(hipHostGetDevicePointer((void**)&d.h_coef_dev, d.h_coef, 0), "pinned device pointer");
         hchk(hipMemcpyAsync(d.H, H, K2 * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
         hchk(hipMemcpyAsync(d.S, S, K2 * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
         hchk(hipMemcpyAsync(d.X, X, K2 * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
@@ -375,7 +391,7 @@ bool DevScf::run(int max_it, int start_diis, bool use_damping, double eps, doubl
             rhs[n] = 1.0;
             hfxh::solve_fullpiv(n + 1, A.data(), rhs.data(), c.data());
             for (int i = 0; i < n; ++i) d.h_coef[i] = c[i];
-            hchk(hipMemcpyAsync(d.coef, d.h_coef, n * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
+            d.copy(d.h_coef_dev, d.coef, (size_t)n);
             k_lincomb<<<grid_for(nK2), 256, 0, d.st>>>(nK2, n, d.coef, d.hF, d.Fuse);
             Fuse = d.Fuse;
         }
@@ -383,12 +399,12 @@ bool DevScf::run(int max_it, int start_diis, bool use_damping, double eps, doubl
         for (int s = 0; s < ns; ++s) {
             d.gemm(K, K, K, 1.0, d.X, K, true, Fuse + (size_t)s * K2, K, false, d.T1, K);
             d.gemm(K, K, K, 1.0, d.T1, K, false, d.X, K, false, d.T2, K);
-            hchk(hipMemcpyAsync(d.h_mat, d.T2, K2 * sizeof(double), hipMemcpyDeviceToHost, d.st), "D2H");
+            d.copy(d.T2, d.h_mat_dev, K2);
             hchk(hipStreamSynchronize(d.st), "sync");
             hfxh::sym_eig(K, d.h_mat, w.data(), Cp.data());
             e_[s] = w;
             for (size_t x = 0; x < K2; ++x) d.h_mat[x] = Cp[x];
-            hchk(hipMemcpyAsync(d.T1, d.h_mat, K2 * sizeof(double), hipMemcpyHostToDevice, d.st), "H2D");
+            d.copy(d.h_mat_dev, d.T1, K2);
             double* Cs = d.C + (size_t)s * K2;
             d.gemm(K, K, K, 1.0, d.X, K, false, d.T1, K, false, Cs, K);
             d.gemm(K, K, d.nocc[s], d.occ, Cs, K, false, Cs, K, true, d.Pn + (size_t)s * K2, K);

@@ -63,3 +63,28 @@ def test_eri_unique_sampled(gpu, name):
     _check_values(got[q], d["eri_sample"], exact, name)
     # 8-fold layout sanity over the whole array: finite, and zero exactly where masked
     assert np.all(np.isfinite(got))
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_class_timing_keeps_values(gpu, mode):
+    """Class timing (1: events on each list's own stream, lists still on four
+    streams; 2: lists serialized on the launch stream) reports a positive time
+    for every launched class and leaves the unique array unchanged."""
+    from hartree_fock_cpp_amd import _native as N
+    d = golden("h2o_ccpvdz.npz")
+    thr = float(d["threshold"])
+    b = gpu.Basis.from_arrays(d)
+    ref = gpu.ERI_GPU(b, thr)
+    N.check(N.lib.hfx_set_class_timing(mode))
+    try:
+        got = gpu.ERI_GPU(b, thr)
+        ms = np.zeros(110, np.float32)
+        N.check(N.lib.hfx_class_times(N.ptr(ms)))
+    finally:
+        N.check(N.lib.hfx_set_class_timing(0))
+    nq = np.zeros(110, np.int64)
+    npq = np.zeros(110, np.int64)
+    N.check(N.lib.hfx_plan_stats(b.handle(), N.C.c_double(thr), 0, N.C.c_double(0.0), N.ptr(nq), N.ptr(npq)))
+    print(f"mode {mode}: {int((ms > 0).sum())} timed lists, {int((nq > 0).sum())} launched, sum {ms.sum():.3f} ms")
+    assert np.all(ms[nq > 0] > 0) and np.all(ms[nq == 0] == 0)
+    assert np.abs(got - ref).max() < 1e-14
