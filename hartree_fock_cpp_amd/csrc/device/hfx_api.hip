@@ -210,6 +210,16 @@ double incremental_tau(const hfx_fock* f) {
     return env >= 0.0 ? env : f->opts.incremental_tau > 0.0 ? f->opts.incremental_tau : 1e-3 * f->opts.threshold;
 }
 
+// Zeroes the J / K accumulators: one launch when they lie back to back (the
+// context's own matrices do), else one per matrix.
+hipError_t zero_jk(double* J, double* Ka, double* Kb, int nspin, size_t K2, hipStream_t st) {
+    if (Ka == J + K2 && (nspin == 1 || Kb == Ka + K2)) return launch_zero(J, (size_t)(1 + nspin) * K2, st);
+    hipError_t e = launch_zero(J, K2, st);
+    if (e == hipSuccess) e = launch_zero(Ka, K2, st);
+    if (e == hipSuccess && nspin == 2) e = launch_zero(Kb, K2, st);
+    return e;
+}
+
 // Relative cost of one list: kept primitive quartets times the work of one
 // (Hermite x Hermite plus Cartesian x Cartesian products of the class).
 double list_cost(const DevLists& dl, int qc, int p) {
@@ -429,13 +439,9 @@ int jk_from_unique(hfx_basis* b, const double* d_eri, const double* d_Dj, const 
                    double* d_J, double* d_Ka, double* d_Kb, int nspin, hipStream_t st) {
     const int K = b->plan.K;
     const size_t K2 = (size_t)K * K;
-    HIPCHK(launch_zero(d_J, K2, st));
-    HIPCHK(launch_zero(d_Ka, K2, st));
-    if (nspin == 2) HIPCHK(launch_zero(d_Kb, K2, st));
+    HIPCHK(zero_jk(d_J, d_Ka, d_Kb, nspin, K2, st));
     HIPCHK(launch_jk_unique(d_eri, b->d_fnpairs, K, nspin, d_Dj, d_Da, d_Db, d_J, d_Ka, d_Kb, st));
-    HIPCHK(launch_symmetrize(d_J, K, st));
-    HIPCHK(launch_symmetrize(d_Ka, K, st));
-    if (nspin == 2) HIPCHK(launch_symmetrize(d_Kb, K, st));
+    HIPCHK(launch_symmetrize_n(d_J, d_Ka, d_Kb, 1 + nspin, K, st));
     return 0;
 }
 
@@ -724,9 +730,7 @@ int enqueue_direct(hfx_fock* f, const double* Dj, const double* d_Pa, const doub
     const int K = b->plan.K;
     const size_t K2 = (size_t)K * K;
     const int nspin = d_Pb ? 2 : 1;
-    HIPCHK(launch_zero(d_J, K2, st));
-    HIPCHK(launch_zero(d_Ka, K2, st));
-    if (nspin == 2) HIPCHK(launch_zero(d_Kb, K2, st));
+    HIPCHK(zero_jk(d_J, d_Ka, d_Kb, nspin, K2, st));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_JK;
     ep.threshold = f->opts.threshold;
@@ -744,9 +748,7 @@ int enqueue_direct(hfx_fock* f, const double* Dj, const double* d_Pa, const doub
         ep.dthr = incremental_tau(f);
     }
     if (int rc = launch_plan(b, f->plan, ep, st)) return rc;
-    HIPCHK(launch_symmetrize(d_J, K, st));
-    HIPCHK(launch_symmetrize(d_Ka, K, st));
-    if (nspin == 2) HIPCHK(launch_symmetrize(d_Kb, K, st));
+    HIPCHK(launch_symmetrize_n(d_J, d_Ka, d_Kb, 1 + nspin, K, st));
     return 0;
 }
 
@@ -893,14 +895,13 @@ int hfx_fock_build(hfx_fock* f, const double* Pa, const double* Pb, double* J, d
     }
     double* h = f->h_stage;
     double* hd = f->h_stage_dev;   // the same block, as the device addresses it
+    // staging [Pa | Pb | J | Ka | Kb] and the context's [Da | Db] and
+    // [J | Ka | Kb] are back to back: one copy launch each way
     std::memcpy(h, Pa, K2 * sizeof(double));
     if (Pb) std::memcpy(h + K2, Pb, K2 * sizeof(double));
-    HIPCHK(launch_copy(f->d_Da, hd, K2, st));
-    if (Pb) HIPCHK(launch_copy(f->d_Db, hd + K2, K2, st));
+    HIPCHK(launch_copy(f->d_Da, hd, (Pb ? 2 : 1) * K2, st));
     if (int rc = hfx_fock_build_device(f, f->d_Da, Pb ? f->d_Db : nullptr, f->d_J, f->d_Ka, f->d_Kb, st)) return rc;
-    if (J) HIPCHK(launch_copy(hd + 2 * K2, f->d_J, K2, st));
-    if (Ka) HIPCHK(launch_copy(hd + 3 * K2, f->d_Ka, K2, st));
-    if (Kb && Pb) HIPCHK(launch_copy(hd + 4 * K2, f->d_Kb, K2, st));
+    HIPCHK(launch_copy(hd + 2 * K2, f->d_J, (Pb ? 3 : 2) * K2, st));
     HIPCHK(hipStreamSynchronize(st));
     if (J) std::memcpy(J, h + 2 * K2, K2 * sizeof(double));
     if (Ka) std::memcpy(Ka, h + 3 * K2, K2 * sizeof(double));

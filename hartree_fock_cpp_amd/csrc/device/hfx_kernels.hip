@@ -141,6 +141,28 @@ __global__ void k_symmetrize(double* M, int K) {
     M[j * K + i] = s;
 }
 
+// The same for up to three matrices in one launch (blockIdx.y picks one).
+__global__ void k_symmetrize3(double* M0, double* M1, double* M2, int K) {
+    double* M = blockIdx.y == 0 ? M0 : blockIdx.y == 1 ? M1 : M2;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = (int64_t)K * (K + 1) / 2;
+    if (p >= np) return;
+    int i = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((int64_t)i * (i + 1) / 2 > p) --i;
+    while ((int64_t)(i + 1) * (i + 2) / 2 <= p) ++i;
+    const int j = (int)(p - (int64_t)i * (i + 1) / 2);
+    const double s = M[i * K + j] + M[j * K + i];
+    M[i * K + j] = s;
+    M[j * K + i] = s;
+}
+
+hipError_t launch_symmetrize_n(double* M0, double* M1, double* M2, int nmat, int K, hipStream_t st) {
+    const int64_t np = (int64_t)K * (K + 1) / 2;
+    const int block = 256;
+    k_symmetrize3<<<dim3((unsigned)((np + block - 1) / block), (unsigned)nmat), block, 0, st>>>(M0, M1, M2, K);
+    return hipGetLastError();
+}
+
 hipError_t launch_symmetrize(double* M, int K, hipStream_t st) {
     const int64_t np = (int64_t)K * (K + 1) / 2;
     const int block = 256;
