@@ -363,6 +363,20 @@ bool DevScf::run(int max_it, int start_diis, bool use_damping, double eps, doubl
     double E_prev = 0.0;
     bool have_prev = false;
     std::vector<double> w(K), Cp(K2);
+    // F' of the last diagonalisation per spin; all_orbitals() solves it again
+    // with every eigenvector for C (the iterations form the occupied ones)
+    std::vector<std::vector<double>> last_Fp(ns);
+    auto all_orbitals = [&]() {
+        for (int s = 0; s < ns; ++s) {
+            if (last_Fp[s].empty()) continue;
+            hfxh::sym_eig(K, last_Fp[s].data(), w.data(), Cp.data());
+            e_[s] = w;
+            for (size_t x = 0; x < K2; ++x) d.h_mat[x] = Cp[x];
+            d.copy(d.h_mat_dev, d.T1, K2);
+            d.gemm(K, K, K, 1.0, d.X, K, false, d.T1, K, false, d.C + (size_t)s * K2, K);
+            hchk(hipStreamSynchronize(d.st), "sync");   // h_mat is reused by the next spin
+        }
+    };
     for (int it = 0; it < max_it; ++it) {
         ++n_it;
         auto t0 = clk::now();
@@ -401,7 +415,10 @@ bool DevScf::run(int max_it, int start_diis, bool use_damping, double eps, doubl
             d.gemm(K, K, K, 1.0, d.T1, K, false, d.X, K, false, d.T2, K);
             d.copy(d.T2, d.h_mat_dev, K2);
             hchk(hipStreamSynchronize(d.st), "sync");
-            hfxh::sym_eig(K, d.h_mat, w.data(), Cp.data());
+            last_Fp[s].assign(d.h_mat, d.h_mat + K2);
+            // occupied orbitals only (all orbital energies); all_orbitals()
+            // forms the rest once the phase ends
+            hfxh::sym_eig_lowest(K, d.h_mat, d.nocc[s], w.data(), Cp.data());
             e_[s] = w;
             for (size_t x = 0; x < K2; ++x) d.h_mat[x] = Cp[x];
             d.copy(d.h_mat_dev, d.T1, K2);
@@ -427,7 +444,7 @@ bool DevScf::run(int max_it, int start_diis, bool use_damping, double eps, doubl
             const double chk = std::sqrt(std::max(0.0, d.h_sc[SC_DOT]));
             if (chk < eps) {
                 hchk(hipMemcpyAsync(d.P, d.Pn, nK2 * sizeof(double), hipMemcpyDeviceToDevice, d.st), "D2D");
-                hchk(hipStreamSynchronize(d.st), "sync");
+                all_orbitals();
                 E_ele = d.h_sc[SC_E];
                 return true;
             }
@@ -443,7 +460,7 @@ bool DevScf::run(int max_it, int start_diis, bool use_damping, double eps, doubl
             if (verbose) std::printf("  last iteration reached\n");
         }
     }
-    hchk(hipStreamSynchronize(d.st), "sync");
+    all_orbitals();
     return false;
 }
 

@@ -44,12 +44,19 @@ namespace {
 // tridiagonal form T = Q^T A Q.  d: diagonal, e[i] = T(i, i+1).  Q is
 // accumulated column-major into Qc (Qc[j*n + i] = Q(i, j)) so that the later
 // Givens updates touch contiguous memory.
+// With refl != nullptr, Q is not formed: the reflector of step k (v over rows
+// k+1..n-1, scale beta[k]; beta 0 = identity) is kept in row k of refl.
 void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::vector<double>& e,
-                    std::vector<double>& Qc) {
+                    std::vector<double>& Qc, std::vector<double>* refl = nullptr, std::vector<double>* betas = nullptr) {
     d.assign(n, 0.0);
     e.assign(n, 0.0);
-    Qc.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) Qc[(size_t)i * n + i] = 1.0;
+    if (refl) {
+        refl->assign((size_t)n * n, 0.0);
+        betas->assign(n, 0.0);
+    } else {
+        Qc.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i) Qc[(size_t)i * n + i] = 1.0;
+    }
     std::vector<double> v(n), p(n), w(n), s(n);
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;   // length of x = A[k+1.., k]
@@ -99,6 +106,11 @@ void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::
             A[(size_t)(k + 1 + i) * n + k] = 0.0;
             A[(size_t)k * n + k + 1 + i] = 0.0;
         }
+        if (refl) {
+            std::copy(v.begin(), v.begin() + m, refl->begin() + (size_t)k * n + k + 1);
+            (*betas)[k] = beta;
+            continue;
+        }
         // Q <- Q H, H = I - beta v v^T acting on columns k+1..n-1: s = Q v, then
         // Q -= beta s v^T, both passes streaming whole columns of Qc
         std::fill(s.begin(), s.end(), 0.0);
@@ -119,8 +131,8 @@ void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::
 }
 
 // Implicit symmetric QR with Wilkinson shift on (d, e); rotations applied to
-// the columns of Qc (column-major).
-void tridiag_qr(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>& Qc) {
+// the columns of Qc (column-major) unless vectors is false.
+void tridiag_qr(int n, std::vector<double>& d, std::vector<double>& e, std::vector<double>& Qc, bool vectors = true) {
     const double eps = 2.220446049250313e-16;
     int iter = 0;
     const int max_iter = 60 * std::max(n, 1);
@@ -168,6 +180,7 @@ void tridiag_qr(int n, std::vector<double>& d, std::vector<double>& e, std::vect
                 z = -sn * e[k + 1];
                 e[k + 1] = cs * e[k + 1];
             }
+            if (!vectors) continue;
             // Q <- Q G (distinct columns: no aliasing, the loop vectorises)
             double* __restrict__ qk = &Qc[(size_t)k * n];
             double* __restrict__ qk1 = &Qc[(size_t)(k + 1) * n];
@@ -202,6 +215,157 @@ void sym_eig(int n, const double* A, double* w, double* V) {
         w[k] = d[src];
         for (int r = 0; r < n; ++r) V[(size_t)r * n + k] = Qc[(size_t)src * n + r];
     }
+}
+
+namespace {
+
+// LU factorisation with partial pivoting of the tridiagonal T - shift I
+// (diagonal d, off-diagonal e) and solves with it (the dgttrf / dgttrs
+// scheme: row interchanges fill a second superdiagonal).  Zero pivots are
+// replaced by `tiny` so that inverse iteration can solve at an eigenvalue.
+struct TriLU {
+    std::vector<double> dl, dd, du, du2;
+    std::vector<int> swap;   // 1 when rows i, i+1 were interchanged
+    int n = 0;
+    void factor(const std::vector<double>& d, const std::vector<double>& e, double shift, double tiny) {
+        n = (int)d.size();
+        dd.resize(n);
+        dl.assign(n, 0.0);
+        du.assign(n, 0.0);
+        du2.assign(n, 0.0);
+        swap.assign(n, 0);
+        for (int i = 0; i < n; ++i) dd[i] = d[i] - shift;
+        for (int i = 0; i + 1 < n; ++i) dl[i] = du[i] = e[i];
+        for (int i = 0; i + 1 < n; ++i) {
+            if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+                if (dd[i] == 0.0) dd[i] = tiny;
+                const double f = dl[i] / dd[i];
+                dl[i] = f;
+                dd[i + 1] -= f * du[i];
+            } else {
+                const double f = dd[i] / dl[i];
+                dd[i] = dl[i];
+                dl[i] = f;
+                const double t = du[i];
+                du[i] = dd[i + 1];
+                dd[i + 1] = t - f * dd[i + 1];
+                if (i + 2 < n) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -f * du[i + 1];
+                }
+                swap[i] = 1;
+            }
+        }
+        if (n > 0 && dd[n - 1] == 0.0) dd[n - 1] = tiny;
+    }
+    void solve(double* b) const {
+        for (int i = 0; i + 1 < n; ++i) {
+            if (swap[i]) {
+                const double t = b[i];
+                b[i] = b[i + 1];
+                b[i + 1] = t - dl[i] * b[i];
+            } else {
+                b[i + 1] -= dl[i] * b[i];
+            }
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double x = b[i];
+            if (i + 1 < n) x -= du[i] * b[i + 1];
+            if (i + 2 < n) x -= du2[i] * b[i + 2];
+            b[i] = x / dd[i];
+        }
+    }
+};
+
+}  // namespace
+
+void sym_eig_lowest(int n, const double* A, int m, double* w, double* V) {
+    m = std::max(0, std::min(m, n));
+    if (n <= 2 || m == n) {
+        sym_eig(n, A, w, V);
+        return;
+    }
+    std::vector<double> a(A, A + (size_t)n * n), d, e, Qc, refl, betas;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            const double s = 0.5 * (a[(size_t)i * n + j] + a[(size_t)j * n + i]);
+            a[(size_t)i * n + j] = s;
+            a[(size_t)j * n + i] = s;
+        }
+    tridiagonalize(n, a, d, e, Qc, &refl, &betas);
+    // eigenvalues: QR without vectors on a copy of T
+    std::vector<double> dq(d), eq(e);
+    tridiag_qr(n, dq, eq, Qc, false);
+    std::sort(dq.begin(), dq.end());
+    for (int k = 0; k < n; ++k) w[k] = dq[k];
+    // eigenvectors of T for the m lowest eigenvalues by inverse iteration,
+    // orthogonalised within clusters (the dstein scheme)
+    double tnorm = 0.0;
+    for (int i = 0; i < n; ++i)
+        tnorm = std::max(tnorm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0));
+    if (tnorm == 0.0) tnorm = 1.0;
+    const double eps = 2.220446049250313e-16;
+    const double ortol = 1e-3 * tnorm, pertol = 10.0 * eps * tnorm;
+    std::vector<double> Z((size_t)m * n), x(n);
+    TriLU lu;
+    int c0 = 0;                 // first index of the current cluster
+    double shift_prev = 0.0;
+    for (int k = 0; k < m; ++k) {
+        double shift = w[k];
+        if (k > 0) {
+            if (w[k] - w[k - 1] > ortol) c0 = k;
+            if (shift - shift_prev < pertol) shift = shift_prev + pertol;
+        }
+        shift_prev = shift;
+        lu.factor(d, e, shift, eps * tnorm);
+        // deterministic start vector
+        unsigned long long r = 0x9E3779B97F4A7C15ull * (unsigned long long)(k + 1);
+        for (int i = 0; i < n; ++i) {
+            r ^= r << 13;
+            r ^= r >> 7;
+            r ^= r << 17;
+            x[i] = (double)(r >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        }
+        for (int it = 0; it < 4; ++it) {
+            double xm = 0.0;
+            for (int i = 0; i < n; ++i) xm = std::max(xm, std::fabs(x[i]));
+            if (xm > 0.0)
+                for (int i = 0; i < n; ++i) x[i] *= 1.0 / xm;
+            lu.solve(x.data());
+            for (int j = c0; j < k; ++j) {   // modified Gram-Schmidt against the cluster
+                const double* z = &Z[(size_t)j * n];
+                double dot = 0.0;
+                for (int i = 0; i < n; ++i) dot += z[i] * x[i];
+                for (int i = 0; i < n; ++i) x[i] -= dot * z[i];
+            }
+            double nrm = 0.0;
+            for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
+            nrm = std::sqrt(nrm);
+            if (nrm == 0.0) {
+                x.assign(n, 0.0);
+                x[k] = 1.0;
+                continue;
+            }
+            for (int i = 0; i < n; ++i) x[i] /= nrm;
+        }
+        std::copy(x.begin(), x.end(), Z.begin() + (size_t)k * n);
+    }
+    // back to A's basis: q = H_0 H_1 ... H_{n-3} z
+    for (int k = 0; k < m; ++k) {
+        double* z = &Z[(size_t)k * n];
+        for (int j = n - 3; j >= 0; --j) {
+            const double beta = betas[j];
+            if (beta == 0.0) continue;
+            const double* v = &refl[(size_t)j * n];
+            double s = 0.0;
+            for (int i = j + 1; i < n; ++i) s += v[i] * z[i];
+            s *= beta;
+            for (int i = j + 1; i < n; ++i) z[i] -= s * v[i];
+        }
+    }
+    for (size_t i = 0; i < (size_t)n * n; ++i) V[i] = 0.0;
+    for (int k = 0; k < m; ++k)
+        for (int r = 0; r < n; ++r) V[(size_t)r * n + k] = Z[(size_t)k * n + r];
 }
 
 void solve_fullpiv(int n, const double* A, const double* b, double* x) {

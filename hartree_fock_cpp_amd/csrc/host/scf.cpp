@@ -112,16 +112,30 @@ struct ScfCore {
     }
 
     // Eigen-solve of X^T F X; returns C = X C' and orbital energies; density.
+    // all_vectors false: only the n_occ occupied columns of C are formed (all
+    // orbital energies still are); the SCF iterations need no more.
     void diag(const std::vector<double>& Fm, std::vector<double>& Cout, std::vector<double>& eout,
-              std::vector<double>& Pout, int n_occ, double occ) {
+              std::vector<double>& Pout, int n_occ, double occ, bool all_vectors = true) {
         const size_t K2 = (size_t)K * K;
         std::vector<double> T1(K2), Fp(K2), Cp(K2);
         gemm(K, X.data(), true, Fm.data(), false, T1.data());
         gemm(K, T1.data(), false, X.data(), false, Fp.data());
         eout.assign(K, 0.0);
-        sym_eig(K, Fp.data(), eout.data(), Cp.data());
         Cout.assign(K2, 0.0);
-        gemm(K, X.data(), false, Cp.data(), false, Cout.data());
+        if (all_vectors) {
+            sym_eig(K, Fp.data(), eout.data(), Cp.data());
+            gemm(K, X.data(), false, Cp.data(), false, Cout.data());
+        } else {
+            sym_eig_lowest(K, Fp.data(), n_occ, eout.data(), Cp.data());
+            for (int i = 0; i < K; ++i) {
+                double* __restrict__ c = &Cout[(size_t)i * K];
+                for (int k = 0; k < K; ++k) {
+                    const double x = X[(size_t)i * K + k];
+                    const double* __restrict__ cp = &Cp[(size_t)k * K];
+                    for (int j = 0; j < n_occ; ++j) c[j] += x * cp[j];
+                }
+            }
+        }
         Pout.assign(K2, 0.0);
         for (int i = 0; i < K; ++i)
             for (int j = 0; j <= i; ++j) {
@@ -198,10 +212,11 @@ struct ScfCore {
             std::vector<std::vector<double>> Pn(nspin);
             C.assign(nspin, {});
             e.assign(nspin, {});
+            last_F.assign(nspin, {});
             double dmax = 0.0;
             for (int s = 0; s < nspin; ++s) {
-                std::vector<double> Fs(Fuse.begin() + s * K2, Fuse.begin() + (s + 1) * K2);
-                diag(Fs, C[s], e[s], Pn[s], nocc[s], nspin == 1 ? 2.0 : 1.0);
+                last_F[s].assign(Fuse.begin() + s * K2, Fuse.begin() + (s + 1) * K2);
+                diag(last_F[s], C[s], e[s], Pn[s], nocc[s], nspin == 1 ? 2.0 : 1.0, false);
                 dmax = std::max(dmax, rmsd(Pn[s], P[s]));
             }
             const double err = vnorm(Rcat);
@@ -217,6 +232,7 @@ struct ScfCore {
                 if (chk < o.epsilon) {
                     P = Pn;
                     E_ele = Ec;
+                    all_orbitals();
                     return true;
                 }
                 if (o.verbose) std::printf("  final check failed, error: %.3e\n", chk);
@@ -233,7 +249,17 @@ struct ScfCore {
                 if (o.verbose) std::printf("  last iteration reached\n");
             }
         }
+        all_orbitals();
         return false;
+    }
+
+    // Extrapolated Fock matrices of the last diagonalisation; the iterations
+    // form only the occupied orbitals, the result gets all of them (the same
+    // eigen-problem solved again with every vector).
+    std::vector<std::vector<double>> last_F;
+    void all_orbitals() {
+        std::vector<double> Pd;
+        for (int s = 0; s < (int)last_F.size(); ++s) diag(last_F[s], C[s], e[s], Pd, 0, 0.0, true);
     }
 };
 

@@ -151,6 +151,15 @@ int hfx_host_sym_eig(int n, const double* A, double* w, double* V) {
     return 0;
 }
 
+int hfx_host_sym_eig_lowest(int n, const double* A, int m, double* w, double* V) {
+    if (n <= 0 || m < 0 || m > n || !A || !w || !V) {
+        hfx_detail::set_error("invalid argument");
+        return HFX_ERR_INVALID;
+    }
+    hfxh::sym_eig_lowest(n, A, m, w, V);
+    return 0;
+}
+
 int hfx_scf_run(int natom, const char* symbols, const double* xyz_atoms, const char* basis_set,
                 const char* basis_path, const hfx_scf_opts* opts, const double* P0a, const double* P0b,
                 hfx_scf_result* res, double* Pa, double* Pb, double* Ca, double* Cb, double* ea, double* eb) {

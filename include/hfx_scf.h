@@ -70,6 +70,12 @@ int hfx_one_electron(int natom, const char* symbols, const double* xyz_atoms, co
  * include/linalg.hpp:138,162): w ascending, V row-major, column k = vector k. */
 int hfx_host_sym_eig(int n, const double* A, double* w, double* V);
 
+/* All eigenvalues (ascending) and the eigenvectors of the m lowest (columns
+ * 0..m-1 of V, the rest zero): what an SCF iteration needs (the occupied
+ * orbitals).  No reference counterpart; inverse iteration on the tridiagonal
+ * form, Gram-Schmidt inside eigenvalue clusters. */
+int hfx_host_sym_eig_lowest(int n, const double* A, int m, double* w, double* V);
+
 /* Runs one SCF.  symbols: natom NUL-separated element symbols; xyz_atoms:
  * 3*natom bohr.  P0a/P0b: optional initial densities (K x K; P0b for UHF).
  * Outputs (each optional): Pa/Pb densities, Ca/Cb coefficients (K x K,

@@ -82,3 +82,51 @@ def test_host_eigensolver_degenerate():
     N.check(N.lib.hfx_host_sym_eig(n, N.ptr(np.ascontiguousarray(A)), N.ptr(w), N.ptr(V)))
     assert np.abs(w - d).max() < 1e-13
     assert np.abs(A @ V - V * w).max() < 1e-12
+
+
+def _lowest(A, m):
+    n = A.shape[0]
+    w, V = np.zeros(n), np.zeros((n, n))
+    N.check(N.lib.hfx_host_sym_eig_lowest(n, N.ptr(np.ascontiguousarray(A)), m, N.ptr(w), N.ptr(V)))
+    return w, V
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (3, 1), (40, 7), (150, 25), (150, 150)])
+def test_host_eigensolver_lowest(n, m):
+    """All eigenvalues and the m lowest eigenvectors (what an SCF iteration
+    uses): eigenvalues as numpy's LAPACK, residual and orthonormality of the
+    m vectors, zero columns beyond m."""
+    rng = np.random.default_rng(n + m)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    w, V = _lowest(A, m)
+    wr = np.linalg.eigvalsh(A)
+    s = max(1, np.abs(wr).max())
+    Vm = V[:, :m]
+    assert np.abs(w - wr).max() < 1e-12 * s
+    assert np.abs(Vm.T @ Vm - np.eye(m)).max() < 1e-12
+    assert np.abs(A @ Vm - Vm * w[:m]).max() < 1e-11 * s
+    assert np.all(V[:, m:] == 0.0)
+
+
+@pytest.mark.parametrize("kind", ["repeated", "block", "identity"])
+def test_host_eigensolver_lowest_clusters(kind):
+    """Degenerate and clustered spectra (orbital shells of symmetric
+    molecules): inverse iteration with Gram-Schmidt inside clusters still
+    returns an orthonormal eigenbasis of every degenerate subspace it cuts."""
+    rng = np.random.default_rng(3)
+    if kind == "repeated":
+        Q, _ = np.linalg.qr(rng.standard_normal((20, 20)))
+        A = Q @ np.diag(np.r_[np.ones(5), 2 * np.ones(5), np.arange(3, 13)]) @ Q.T
+    elif kind == "block":
+        B = rng.standard_normal((10, 10))
+        A = np.kron(np.eye(4), B + B.T)
+    else:
+        A = np.eye(12)
+    n = A.shape[0]
+    for m in (1, 3, n // 2):
+        w, V = _lowest(A, m)
+        Vm = V[:, :m]
+        assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * max(1, np.abs(w).max())
+        assert np.abs(Vm.T @ Vm - np.eye(m)).max() < 1e-12
+        assert np.abs(A @ Vm - Vm * w[:m]).max() < 1e-12 * max(1, np.abs(w).max())
