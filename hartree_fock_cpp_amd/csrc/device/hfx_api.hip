@@ -180,6 +180,18 @@ struct hfx_fock {
     bool have_prev = false;
     int prev_nspin = 0;
     int since_full = 0;
+    // work-unit autotuner of direct builds: the first NTUNE builds each run a
+    // full (non-incremental, eager) build with one candidate slice target,
+    // timed with HIP events; the fastest plan is kept for the rest
+    static constexpr int NTUNE = 3;
+    bool tuning = false;
+    bool tune_warm = true;           // the first build (kernel loading) is not timed
+    int tune_i = 0;                  // candidate of the next / pending timed build
+    bool tune_pending = false;
+    long long tune_target[NTUNE] = {};
+    float tune_ms[NTUNE] = {};
+    long long plan_target = 0;       // slice target of the current plan
+    hipEvent_t tune_ev[2] = {};
 };
 
 namespace {
@@ -318,6 +330,21 @@ long long slice_target() {
 struct SliceTargets {
     long long fam_a, fam_b;
 };
+
+// Work-unit autotuning of direct builds (hfx_set_autotune; HFX_AUTOTUNE=1 in
+// the environment turns it on).  Off by default: measured on the n-octane and
+// (H2O)4 direct SCFs (~30 builds), three full, re-planned tuning builds cost
+// more than the plan they find saves.
+std::atomic<int> g_autotune{-1};
+bool autotune_on() {
+    int v = g_autotune.load();
+    if (v < 0) {
+        const char* e = std::getenv("HFX_AUTOTUNE");
+        v = (e && e[0] == '1') ? 1 : 0;
+        g_autotune.store(v);
+    }
+    return v == 1;
+}
 SliceTargets jk_slices() { return {slice_target(), slice_target()}; }
 SliceTargets unique_slices() { return {0, slice_target()}; }
 
@@ -645,6 +672,14 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
         rc = fail(HFX_ERR_DEVICE, "hipEventCreate");
     if (!rc && opts->incremental && mode == HFX_MODE_DIRECT)
         rc = dalloc(&f->d_inc, 8 * K2 + (size_t)b->plan.nshell * b->plan.nshell);
+    f->plan_target = slice_target();
+    if (!rc && mode == HFX_MODE_DIRECT && autotune_on() && f->plan_target > 0) {
+        f->tune_target[0] = f->plan_target;   // the plan just made
+        f->tune_target[1] = std::max<long long>(1, f->plan_target / 4);
+        f->tune_target[2] = f->plan_target * 4;
+        if (hipEventCreate(&f->tune_ev[0]) == hipSuccess && hipEventCreate(&f->tune_ev[1]) == hipSuccess)
+            f->tuning = true;
+    }
     if (!rc) {
         f->d_Dj = buf;
         f->d_Da = buf + K2;
@@ -675,6 +710,8 @@ void hfx_fock_destroy(hfx_fock* f) {
     }
     if (f->b && f->b->stream) (void)hipStreamSynchronize(f->b->stream);
     if (f->graph) (void)hipGraphExecDestroy(f->graph);
+    for (hipEvent_t e : f->tune_ev)
+        if (e) (void)hipEventDestroy(e);
     dfree(f->d_eri);
     dfree(f->d_inc);
     if (f->h_stage) hfx_detail::host_free(f->h_stage);
@@ -688,6 +725,22 @@ int hfx_fock_info(const hfx_fock* f, int* mode, size_t* device_bytes, size_t* n_
     if (mode) *mode = f->mode;
     if (device_bytes) *device_bytes = f->bytes;
     if (n_quartets) *n_quartets = f->nq;
+    return 0;
+}
+
+int hfx_set_autotune(int on) {
+    g_autotune.store(on ? 1 : 0);
+    return 0;
+}
+
+int hfx_fock_autotune_info(const hfx_fock* f, int* done, long long* target, long long* targets, float* ms) {
+    if (!f) return fail(HFX_ERR_INVALID, "null fock");
+    if (done) *done = f->tune_target[0] != 0 && !f->tuning ? 1 : 0;
+    if (target) *target = f->plan_target;
+    for (int i = 0; i < hfx_fock::NTUNE; ++i) {
+        if (targets) targets[i] = f->tune_target[i];
+        if (ms) ms[i] = f->tune_ms[i];
+    }
     return 0;
 }
 
@@ -822,11 +875,78 @@ int enqueue_incremental(hfx_fock* f, const double* d_Pa, const double* d_Pb, dou
 
 }  // namespace
 
+namespace {
+
+// Re-plans a direct context's work units for slice target t; the previous
+// plan's builds must have completed (its lists go back to the cache).
+int replan(hfx_fock* f, long long t) {
+    if (f->graph) {
+        HIPCHK(hipGraphExecDestroy(f->graph));
+        f->graph = nullptr;
+    }
+    if (int rc = make_plan(f->b, f->opts.threshold, f->opts.precision == HFX_PREC_MIXED, f->opts.mixed_split,
+                           SliceTargets{t, t}, f->plan))
+        return rc;
+    f->plan_target = t;
+    return 0;
+}
+
+// One autotuner step at the start of a direct build: collects the timing of
+// the previous tuning build; returns 1 when this build is a tuning build
+// (then it is enqueued full and eager between the tuning events by the
+// caller), 0 otherwise (the tuner finished: the fastest plan is in place).
+int tune_step(hfx_fock* f, int& is_tuning) {
+    is_tuning = 0;
+    if (!f->tuning) return 0;
+    if (f->tune_warm) {
+        f->tune_warm = false;
+        return 0;
+    }
+    if (f->tune_pending) {
+        HIPCHK(hipEventSynchronize(f->tune_ev[1]));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, f->tune_ev[0], f->tune_ev[1]));
+        f->tune_ms[f->tune_i] = ms;
+        f->tune_pending = false;
+        ++f->tune_i;
+    }
+    if (f->tune_i < hfx_fock::NTUNE) {
+        if (f->plan_target != f->tune_target[f->tune_i])
+            if (int rc = replan(f, f->tune_target[f->tune_i])) return rc;
+        f->have_prev = false;   // a full build, comparable across candidates
+        is_tuning = 1;
+        return 0;
+    }
+    int best = 0;
+    for (int i = 1; i < hfx_fock::NTUNE; ++i)
+        if (f->tune_ms[i] < f->tune_ms[best]) best = i;
+    if (f->plan_target != f->tune_target[best])
+        if (int rc = replan(f, f->tune_target[best])) return rc;
+    f->tuning = false;
+    return 0;
+}
+
+}  // namespace
+
 int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka,
                           double* d_Kb, void* stream) {
     if (!f || !d_Pa || !d_J || !d_Ka) return fail(HFX_ERR_INVALID, "null argument");
     if (d_Pb && !d_Kb) return fail(HFX_ERR_INVALID, "UHF build needs Kb");
     hipStream_t st = stream ? (hipStream_t)stream : f->b->stream;
+    if (f->mode == HFX_MODE_DIRECT && !g_timer.on) {
+        int is_tuning = 0;
+        if (int rc = tune_step(f, is_tuning)) return rc;
+        if (is_tuning) {
+            HIPCHK(hipEventRecord(f->tune_ev[0], st));
+            const int rc = f->d_inc ? enqueue_incremental(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st)
+                                    : enqueue_build(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(f->tune_ev[1], st));
+            f->tune_pending = true;
+            HIPCHK(hipEventRecord(f->ev_done, st));
+            return 0;
+        }
+    }
     if (f->d_inc && !g_timer.on) {   // incremental direct builds: eager
         if (int rc = enqueue_incremental(f, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st)) return rc;
         HIPCHK(hipEventRecord(f->ev_done, st));

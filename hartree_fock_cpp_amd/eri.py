@@ -107,6 +107,14 @@ class FockBuilder:
         N.check(N.lib.hfx_fock_info(self._h, C.byref(m), C.byref(b), C.byref(n)))
         return {"mode": "stored" if m.value == 0 else "direct", "device_bytes": b.value, "n_quartets": n.value}
 
+    def autotune_info(self):
+        """dict(done, target, targets, ms) of the direct-build work-unit autotuner."""
+        done, tgt = C.c_int(), C.c_longlong()
+        targets = np.zeros(3, np.int64)
+        ms = np.zeros(3, np.float32)
+        N.check(N.lib.hfx_fock_autotune_info(self._h, C.byref(done), C.byref(tgt), N.ptr(targets), N.ptr(ms)))
+        return {"done": bool(done.value), "target": tgt.value, "targets": targets.tolist(), "ms": ms.tolist()}
+
     def class_counts(self):
         n64 = np.zeros(55, np.int64)
         n32 = np.zeros(55, np.int64)

@@ -140,6 +140,19 @@ int hfx_fock_build_device(hfx_fock* f, const double* d_Pa, const double* d_Pb, d
  * ss,ps,pp,ds,dp,dd,fs,fp,fd,ff), the number of FP64 and FP32 shell quartets. */
 int hfx_fock_class_counts(const hfx_fock* f, long long* n_fp64, long long* n_fp32);
 
+/* Work-unit autotuner of a direct context (no reference counterpart): after
+ * an untimed first build, the next three builds run in full with the slice
+ * target T, T/4 and 4T (T = 2048 wavefronts, or HFX_SLICE_TARGET), timed with
+ * HIP events; the fastest plan serves the later builds.  Off by default
+ * (hfx_set_autotune(1) or HFX_AUTOTUNE=1 turns it on for contexts created
+ * afterwards): for an SCF of ~30 builds the tuning builds cost more than they
+ * save; it pays for long sequences of builds on one context.  Results do not
+ * depend on the plan beyond summation order.  done: 1 once chosen; target:
+ * the current plan's; targets / ms: the candidates and their measured build
+ * times (0 while not measured). */
+int hfx_set_autotune(int on);
+int hfx_fock_autotune_info(const hfx_fock* f, int* done, long long* target, long long* targets, float* ms);
+
 /* Measurement hooks (no reference counterpart).  With class timing on, every
  * ERI class launch is bracketed by HIP events on its stream; hfx_class_times
  * returns the last build's per-launch milliseconds, ms[2*qc + p] with p = 0

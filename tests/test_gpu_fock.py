@@ -127,3 +127,27 @@ def test_incremental_matches_full_builds(gpu, uhf):
     Jref = orc.coulomb_matrix(Pa, Pb if uhf else np.zeros_like(Pa), eri)
     print(f"incremental vs full: max diff {worst:.2e}; J vs oracle {np.abs(J - Jref).max():.2e}")
     assert np.abs(J - Jref).max() < 1e-9
+
+
+def test_direct_autotuner_keeps_results(gpu):
+    """The work-unit autotuner of direct builds tries three slice targets on
+    builds 2-4 and keeps the fastest; every build's J/K equals the first
+    (summation order only) and the chosen target is one of the candidates."""
+    from hartree_fock_cpp_amd import _native as N
+    d = golden("h2o_ccpvdz.npz")
+    K = int(d["K"])
+    b = gpu.Basis.from_arrays(d)
+    N.check(N.lib.hfx_set_autotune(1))
+    try:
+        fb = gpu.FockBuilder(b, mode="direct", threshold=1e-12)
+    finally:
+        N.check(N.lib.hfx_set_autotune(0))
+    P = _dens(K, 21)
+    J0, K0 = fb.build(P)
+    for _ in range(5):
+        J, Kx = fb.build(P)
+        assert np.abs(J - J0).max() < 1e-12 and np.abs(Kx - K0).max() < 1e-12
+    info = fb.autotune_info()
+    print("autotune:", info)
+    assert info["done"] and info["target"] in info["targets"]
+    assert all(m > 0 for m in info["ms"])
