@@ -104,6 +104,23 @@ def test_mixed_precision_with_polish(gpu):
     assert np.abs(rmx["density"] - r64["density"]).max() < 1e-6
 
 
+def test_mixed_stored_scf_with_polish(gpu):
+    """The mixed form in stored mode (FP32-evaluated small-bound quartets kept
+    in the FP64 unique array): within 1e-6 Ha of FP64 before the polish and
+    1e-9 Ha after it."""
+    from hartree_fock_cpp_amd.scf import rhf_diis
+    d = golden("h2o_ccpvdz.npz")
+    cfg = _config(d)
+    r64 = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-10, eri_mode="stored", screening_threshold=1e-12)
+    rmx = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-10, eri_mode="stored", screening_threshold=1e-12,
+                   precision="mixed", mixed_split=1e-3)
+    print(f"stored mixed: before polish dE={rmx['E_before_polish'] - r64['E_tot']:.2e}, "
+          f"after dE={rmx['E_tot'] - r64['E_tot']:.2e}, mode {rmx['mode']}")
+    assert rmx["converged"] and rmx["mode"] == 0
+    assert abs(rmx["E_before_polish"] - r64["E_tot"]) < 1e-6
+    assert abs(rmx["E_tot"] - r64["E_tot"]) < 1e-9
+
+
 def test_incremental_fock_scf_matches_full(gpu):
     """Direct SCF with incremental Fock builds converges to the energy and
     density of the full-build SCF (host and GPU-resident iterations)."""
