@@ -9,6 +9,8 @@
 // ids, so one launch covers one angular class and one precision.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "eri_body.hpp"
 #include "eri_classes.hpp"
 #include "hfx_launch.hpp"
@@ -47,9 +49,10 @@ bool class_uses_family_a(int qc) {
 // (rows longest-first).  Row elements (ij|kl), KL <= IJ, are read once,
 // coalesced.  With f the function-level degeneracy factor:
 //   Jacc_ij += 2f v Dj_kl   (block reduction, one atomic per row)
-//   Jacc_kl += 2f v Dj_ij   (one atomic per element)
 //   Kacc_ik += f v D_jl, Kacc_il += f v D_jk  (row i, LDS accumulator)
 //   Kacc_jk += f v D_il, Kacc_jl += f v D_ik  (row j, LDS accumulator)
+// The column terms Jacc_kl += 2f v Dj_ij are k_j_columns' (a per-element
+// atomic here would cost one global atomic per stored integral).
 constexpr int JK_BLOCK = 256;
 constexpr int JK_MAXK = 900;   // 2 spins x 4 rows x K doubles must fit 64 KiB of LDS
 
@@ -86,7 +89,6 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk_unique(const double* __restrict
         const double f = fij * (k == l ? 0.5 : 1.0) * (KL == IJ ? 0.5 : 1.0);
         const double fv = f * v;
         srow += 2.0 * fv * Dj[k * K + l];
-        unsafeAtomicAdd(&J[k * K + l], 2.0 * fv * dij);
         for (int s = 0; s < nspin; ++s) {
             double* base = lds + s * 4 * K;
             const double* Di = base + 2 * K;
@@ -117,12 +119,47 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk_unique(const double* __restrict
     }
 }
 
+// Column terms of J from the unique array: Jacc_kl += sum over rows IJ >= KL
+// of 2 f v(IJ, KL) Dj_ij.  A workgroup takes a block of JC_COLS columns (one
+// per thread) against a chunk of `rows` rows, reads each row's segment of
+// the block (coalesced) and adds its column sums with one atomic per column:
+// (P / rows) x P atomics instead of one per stored integral.  rows shrinks
+// with P (16 .. 256) so small matrices still spread over many workgroups.
+constexpr int JC_COLS = 256;
+
+__global__ __launch_bounds__(JC_COLS) void k_j_columns(const double* __restrict__ eri, const int2* __restrict__ pairs,
+                                                       int K, int npair, int rows, const double* __restrict__ Dj,
+                                                       double* J) {
+    const int KL = (int)blockIdx.x * JC_COLS + (int)threadIdx.x;
+    const int col0 = (int)blockIdx.x * JC_COLS;
+    const int r_lo = max((int)blockIdx.y * rows, col0);
+    const int r_hi = min((int)(blockIdx.y + 1) * rows, npair);
+    if (r_lo >= r_hi) return;   // uniform: the chunk lies above the block's diagonal
+    double acc = 0.0;
+    for (int IJ = r_lo; IJ < r_hi; ++IJ) {
+        const int2 ij = pairs[IJ];
+        const double rowf = 2.0 * (ij.x == ij.y ? 0.5 : 1.0) * Dj[ij.x * K + ij.y];
+        if (KL <= IJ) {
+            const double v = eri[(size_t)IJ * (IJ + 1) / 2 + KL];
+            acc += (KL == IJ ? 0.5 : 1.0) * rowf * v;
+        }
+    }
+    if (KL < npair && acc != 0.0) {
+        const int2 kl = pairs[KL];
+        unsafeAtomicAdd(&J[kl.x * K + kl.y], (kl.x == kl.y ? 0.5 : 1.0) * acc);
+    }
+}
+
 hipError_t launch_jk_unique(const double* eri, const int2* pairs, int K, int nspin, const double* Dj, const double* Da,
                             const double* Db, double* J, double* Ka, double* Kb, hipStream_t st) {
     if (K > JK_MAXK) return hipErrorInvalidValue;
     const int npair = K * (K + 1) / 2;
     const size_t lds = sizeof(double) * (size_t)(nspin * 4 * K + JK_BLOCK / 64);
     k_jk_unique<<<npair, JK_BLOCK, lds, st>>>(eri, pairs, K, npair, nspin, Dj, Da, Db, J, Ka, Kb);
+    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+    const int rows = std::min(256, std::max(16, npair / 64));
+    const dim3 grid((unsigned)((npair + JC_COLS - 1) / JC_COLS), (unsigned)((npair + rows - 1) / rows));
+    k_j_columns<<<grid, JC_COLS, 0, st>>>(eri, pairs, K, npair, rows, Dj, J);
     return hipGetLastError();
 }
 
