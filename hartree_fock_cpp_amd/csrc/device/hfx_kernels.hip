@@ -51,10 +51,64 @@ bool class_uses_family_a(int qc) {
 //   Jacc_ij += 2f v Dj_kl   (block reduction, one atomic per row)
 //   Kacc_ik += f v D_jl, Kacc_il += f v D_jk  (row i, LDS accumulator)
 //   Kacc_jk += f v D_il, Kacc_jl += f v D_ik  (row j, LDS accumulator)
+// Pairs are ordered k-major, so the lanes of a wave hold a few runs of equal
+// k: the k-keyed terms are summed over each run with a segmented wave scan
+// (DPP lane shifts, no LDS traffic) and added by the run's last lane, one LDS
+// atomic per run instead of 64 atomics on one address; the l-keyed terms hit
+// distinct addresses.  (k, l) come from KL arithmetically, so the gather of
+// Dj_kl issues with the integral load, and each thread keeps JK_UNROLL
+// elements in flight.
 // The column terms Jacc_kl += 2f v Dj_ij are k_j_columns' (a per-element
 // atomic here would cost one global atomic per stored integral).
 constexpr int JK_BLOCK = 256;
+constexpr int JK_UNROLL = 4;
 constexpr int JK_MAXK = 900;   // 2 spins x 4 rows x K doubles must fit 64 KiB of LDS
+
+// (k, l), k >= l, of the k-major pair index p = k(k+1)/2 + l (p < 2^21, so
+// 8p + 1 is exact in float; the two corrections absorb the rounding of sqrtf).
+__device__ inline void pair_of(int p, int& k, int& l) {
+    int r = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+    if ((r + 1) * (r + 2) / 2 <= p) ++r;
+    if (r * (r + 1) / 2 > p) --r;
+    k = r;
+    l = p - r * (r + 1) / 2;
+}
+
+// value of lane (lane - shift) under DPP control CTRL, 0 where that lane lies
+// outside the row or the destination row is masked off by RM
+template <int CTRL, int RM>
+__device__ inline int dpp_int(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ inline double dpp_dbl(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_int<CTRL, RM>((int)b);
+    const int hi = dpp_int<CTRL, RM>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int RM>
+__device__ inline void run_step(double& a, double& b, int key) {
+    const int ok = dpp_int<CTRL, RM>(key);
+    const double oa = dpp_dbl<CTRL, RM>(a), ob = dpp_dbl<CTRL, RM>(b);
+    if (ok == key) {
+        a += oa;
+        b += ob;
+    }
+}
+// Inclusive scan of (a, b) over runs of equal key (key >= 1, nondecreasing
+// over the wave's lanes; a shifted-in 0 never matches): row_shr 1, 2, 4, 8
+// within each 16-lane row, then row_bcast 15 (rows 1, 3) and row_bcast 31
+// (rows 2, 3).  Afterwards the last lane of a run holds the run sums.  All
+// 64 lanes must be active.
+__device__ inline void run_scan2(double& a, double& b, int key) {
+    run_step<0x111, 0xf>(a, b, key);
+    run_step<0x112, 0xf>(a, b, key);
+    run_step<0x114, 0xf>(a, b, key);
+    run_step<0x118, 0xf>(a, b, key);
+    run_step<0x142, 0xa>(a, b, key);
+    run_step<0x143, 0xc>(a, b, key);
+}
 
 __global__ __launch_bounds__(JK_BLOCK) void k_jk_unique(const double* __restrict__ eri, const int2* __restrict__ pairs,
                                                          int K, int npair, int nspin, const double* __restrict__ Dj,
@@ -65,7 +119,7 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk_unique(const double* __restrict
     const int IJ = npair - 1 - (int)blockIdx.x;
     const int2 ij = pairs[IJ];
     const int i = ij.x, j = ij.y;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     for (int s = 0; s < nspin; ++s) {
         double* base = lds + s * 4 * K;
         const double* D = s == 0 ? Da : Db;
@@ -77,32 +131,54 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk_unique(const double* __restrict
         }
     }
     __syncthreads();
-    const double dij = Dj[i * K + j];
     const double fij = (i == j) ? 0.5 : 1.0;
     const size_t rowbase = (size_t)IJ * (IJ + 1) / 2;
     double srow = 0.0;
-    for (int KL = tid; KL <= IJ; KL += JK_BLOCK) {
-        const double v = eri[rowbase + KL];
-        if (v == 0.0) continue;
-        const int2 kl = pairs[KL];
-        const int k = kl.x, l = kl.y;
-        const double f = fij * (k == l ? 0.5 : 1.0) * (KL == IJ ? 0.5 : 1.0);
-        const double fv = f * v;
-        srow += 2.0 * fv * Dj[k * K + l];
-        for (int s = 0; s < nspin; ++s) {
-            double* base = lds + s * 4 * K;
-            const double* Di = base + 2 * K;
-            const double* Dj_ = base + 3 * K;
-            atomicAdd(&base[k], fv * Dj_[l]);          // Kacc_ik += f v D_jl
-            atomicAdd(&base[l], fv * Dj_[k]);          // Kacc_il += f v D_jk
-            atomicAdd(&base[K + k], fv * Di[l]);       // Kacc_jk += f v D_il
-            atomicAdd(&base[K + l], fv * Di[k]);       // Kacc_jl += f v D_ik
+    // the trip count is uniform over the block: every lane takes part in the scans
+    for (int KL0 = 0; KL0 <= IJ; KL0 += JK_BLOCK * JK_UNROLL) {
+        double v[JK_UNROLL], dkl[JK_UNROLL];
+        int kk[JK_UNROLL], ll[JK_UNROLL];
+#pragma unroll
+        for (int u = 0; u < JK_UNROLL; ++u) {
+            const int KL = KL0 + u * JK_BLOCK + tid;
+            const bool in = KL <= IJ;
+            int k, l;
+            pair_of(in ? KL : 0, k, l);
+            kk[u] = in ? k : 0x3fffffff;   // lanes past the row form a run of their own
+            ll[u] = l;
+            v[u] = in ? eri[rowbase + KL] : 0.0;
+            dkl[u] = in ? Dj[k * K + l] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < JK_UNROLL; ++u) {
+            const int KL = KL0 + u * JK_BLOCK + tid;
+            const int k = kk[u], l = ll[u];
+            const double fv = fij * (k == l ? 0.5 : 1.0) * (KL == IJ ? 0.5 : 1.0) * v[u];
+            srow += 2.0 * fv * dkl[u];
+            const bool last = lane == 63 || l == k || KL == IJ;
+            for (int s = 0; s < nspin; ++s) {
+                double* base = lds + s * 4 * K;
+                const double* Di = base + 2 * K;
+                const double* Dj_ = base + 3 * K;
+                double a = 0.0, b = 0.0;
+                if (fv != 0.0) {
+                    a = fv * Dj_[l];                        // Kacc_ik += f v D_jl
+                    b = fv * Di[l];                         // Kacc_jk += f v D_il
+                    atomicAdd(&base[l], fv * Dj_[k]);       // Kacc_il += f v D_jk
+                    atomicAdd(&base[K + l], fv * Di[k]);    // Kacc_jl += f v D_ik
+                }
+                run_scan2(a, b, k + 1);
+                if (last && k < K) {
+                    if (a != 0.0) atomicAdd(&base[k], a);
+                    if (b != 0.0) atomicAdd(&base[K + k], b);
+                }
+            }
         }
     }
     // block reduction of the row sum
     for (int off = 32; off > 0; off >>= 1) srow += __shfl_down(srow, off, 64);
     double* red = lds + nspin * 4 * K;
-    if ((tid & 63) == 0) red[tid >> 6] = srow;
+    if (lane == 0) red[tid >> 6] = srow;
     __syncthreads();
     if (tid == 0) {
         double s = 0.0;
