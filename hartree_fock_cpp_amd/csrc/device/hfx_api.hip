@@ -352,29 +352,50 @@ SliceTargets unique_slices() { return {0, slice_target()}; }
 // wavefront targets per family; 0 for the Schwarz diagonal, whose epilogue
 // needs whole quartets) into one device buffer; fills offsets / counts /
 // buckets in units.
+// Expands each (class, precision) list into work units (lists in parallel on
+// large plans) and uploads them in list order through pinned staging.
 int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target, DevLists& dl) {
-    std::vector<uint2> all;
-    all.reserve(ql.total());
-    for (int qc = 0; qc < NQCLASS; ++qc)
-        for (int p = 0; p < 2; ++p) {
-            dl.off[qc][p] = all.size();
-            const std::vector<uint2>& v = ql.list[qc][p];
-            size_t pos = 0;
-            for (int g = 0; g < NGROUP; ++g) {
-                const size_t before = all.size();
-                expand_units(b->plan, qc_family_a(qc), GROUP_SIZE[g], v.data() + pos, ql.bucket[qc][p][g],
-                             qc_family_a(qc) ? target.fam_a : target.fam_b, all, qc_famb_kb(qc));
-                pos += ql.bucket[qc][p][g];
-                dl.bucket[qc][p][g] = (int)(all.size() - before);
-            }
-            dl.cnt[qc][p] = (int)(all.size() - dl.off[qc][p]);
-            dl.nquart[qc][p] = (long long)v.size();
+    constexpr int NL = 2 * NQCLASS;
+    const int T = plan_threads(ql.total());
+    std::vector<std::vector<uint2>> units(NL);
+    parallel_tasks(NL, T, [&](int l) {
+        const int qc = l / 2, p = l % 2;
+        const std::vector<uint2>& v = ql.list[qc][p];
+        std::vector<uint2>& u = units[l];
+        u.reserve(v.size());
+        size_t pos = 0;
+        for (int g = 0; g < NGROUP; ++g) {
+            const size_t before = u.size();
+            expand_units(b->plan, qc_family_a(qc), GROUP_SIZE[g], v.data() + pos, ql.bucket[qc][p][g],
+                         qc_family_a(qc) ? target.fam_a : target.fam_b, u, qc_famb_kb(qc));
+            pos += ql.bucket[qc][p][g];
+            dl.bucket[qc][p][g] = (int)(u.size() - before);
         }
-    if (int rc = dalloc(&dl.d, all.size())) return rc;
-    if (!all.empty()) {
-        HIPCHK(hipMemcpyAsync(dl.d, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
-        HIPCHK(hipStreamSynchronize(b->stream));
+        long long s = 0;
+        for (const uint2& e : v) s += (long long)b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim;
+        dl.nprimq[qc][p] = s;
+    });
+    size_t total = 0;
+    for (int l = 0; l < NL; ++l) {
+        const int qc = l / 2, p = l % 2;
+        dl.off[qc][p] = total;
+        dl.cnt[qc][p] = (int)units[l].size();
+        dl.nquart[qc][p] = (long long)ql.list[qc][p].size();
+        total += units[l].size();
     }
+    if (int rc = dalloc(&dl.d, total)) return rc;
+    if (total == 0) return 0;
+    void* h = nullptr;
+    HIPCHK(hfx_detail::host_alloc(&h, total * sizeof(uint2)));
+    uint2* hs = static_cast<uint2*>(h);
+    parallel_tasks(NL, T, [&](int l) {
+        if (!units[l].empty())
+            std::memcpy(hs + dl.off[l / 2][l % 2], units[l].data(), units[l].size() * sizeof(uint2));
+    });
+    hipError_t e = hipMemcpyAsync(dl.d, hs, total * sizeof(uint2), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    hfx_detail::host_free(h);
+    if (e != hipSuccess) return fail(HFX_ERR_DEVICE, std::string("quartet list upload: ") + hipGetErrorString(e));
     return 0;
 }
 
@@ -382,16 +403,11 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target
 // for the FP64 unique build).
 int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets target, DevLists& dl) {
     dfree(dl.d);
+    dl.d = nullptr;
     dl.valid = false;
     QuartetLists ql;
     plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql);
     if (int rc = upload_lists(b, ql, target, dl)) return rc;
-    for (int qc = 0; qc < NQCLASS; ++qc)
-        for (int p = 0; p < 2; ++p) {
-            long long s = 0;
-            for (const uint2& e : ql.list[qc][p]) s += (long long)b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim;
-            dl.nprimq[qc][p] = s;
-        }
     dl.threshold = thr;
     dl.mixed = mixed;
     dl.split = split;
@@ -651,12 +667,24 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     if (mode == HFX_MODE_AUTO)
         mode = (opts->hbm_budget == 0 || stored_bytes <= opts->hbm_budget) ? HFX_MODE_STORED : HFX_MODE_DIRECT;
     // direct: the J/K plan; stored mixed: the plan of its unique array (FP32
-    // for the small-bound quartets); stored FP64 only counts quartets here and
-    // builds from the basis' cached unique plan
-    if (int rc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split,
-                           mode == HFX_MODE_STORED ? unique_slices() : jk_slices(), f->plan)) {
+    // for the small-bound quartets); stored FP64 builds from the basis' cached
+    // unique plan (made here if missing) and keeps only its counts
+    int prc = 0;
+    if (mode == HFX_MODE_STORED && !mixed) {
+        DevLists& u = b->unique_plan;
+        if (!u.valid || u.threshold != opts->threshold)
+            prc = make_plan(b, opts->threshold, 0, 0.0, unique_slices(), u);
+        if (!prc) {
+            f->plan = u;
+            f->plan.d = nullptr;   // the lists stay the basis'
+        }
+    } else {
+        prc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split,
+                        mode == HFX_MODE_STORED ? unique_slices() : jk_slices(), f->plan);
+    }
+    if (prc) {
         hfx_fock_destroy(f);
-        return rc;
+        return prc;
     }
     for (int qc = 0; qc < NQCLASS; ++qc) f->nq += f->plan.nquart[qc][0] + f->plan.nquart[qc][1];
     const size_t direct_bytes = f->nq * sizeof(uint2) + mats;

@@ -5,9 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "eri_body.hpp"
@@ -204,6 +207,40 @@ struct QuartetLists {
     }
 };
 
+// Host threads for planning: one below 2^18 work items (thread start-up
+// would cost more than it saves), else up to 16.
+inline int plan_threads(size_t work) {
+    if (work < ((size_t)1 << 18)) return 1;
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::clamp<unsigned>(hc ? hc : 1u, 1u, 16u);
+}
+
+// fn(task) for every task in [0, ntask), spread over up to nthreads threads
+// (the caller's thread included) by a shared counter.  fn must not throw.
+// If a thread cannot be started, the threads already running take its tasks.
+template <class F>
+inline void parallel_tasks(int ntask, int nthreads, F&& fn) {
+    if (nthreads <= 1 || ntask <= 1) {
+        for (int t = 0; t < ntask; ++t) fn(t);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto worker = [&] {
+        for (int t; (t = next.fetch_add(1)) < ntask;) fn(t);
+    };
+    std::vector<std::thread> th;
+    const int n = std::min(nthreads, ntask);
+    for (int i = 1; i < n; ++i) {
+        try {
+            th.emplace_back(worker);
+        } catch (...) {
+            break;
+        }
+    }
+    worker();
+    for (auto& x : th) x.join();
+}
+
 // Screened canonical shell quartets (x >= y over canonical pair ids), split by
 // class and precision.  A quartet is kept when Q_x Q_y >= threshold (no
 // function quartet inside it can pass the function-level mask otherwise).
@@ -213,9 +250,13 @@ struct QuartetLists {
 //
 // Two passes over the pair triangle (count, then place) and a stable counting
 // sort on w per list: linear in the number of quartets (13.6 M for
-// n-octane/cc-pVDZ at 1e-12), with no per-element reallocation.
+// n-octane/cc-pVDZ at 1e-12), with no per-element reallocation.  Large
+// triangles are cut into row ranges of equal area, one per thread; a thread's
+// entries of a list follow those of the threads before it, so the lists come
+// out in the same order as from one thread.
 inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, double thr, bool mixed, double split,
                           QuartetLists& ql) {
+    constexpr int NL = 2 * NQCLASS;
     const int np = bp.npair;
     std::vector<int> nprim(np), pcls(np);
     for (int x = 0; x < np; ++x) {
@@ -233,45 +274,63 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
         ket = (unsigned)k;
         return 2 * quartet_class(pcls[b], pcls[k]) + ((mixed && qq < split) ? 1 : 0);
     };
-    std::vector<size_t> cnt(2 * NQCLASS, 0);
-    for (int x = 0; x < np; ++x)
-        for (int y = 0; y <= x; ++y) {
-            unsigned b, k;
-            const int l = list_of(x, y, b, k);
-            if (l >= 0) ++cnt[l];
+    const int T = plan_threads((size_t)np * (np + 1) / 2);
+    std::vector<int> row0(T + 1);   // thread t takes rows [row0[t], row0[t+1])
+    for (int t = 0; t <= T; ++t) row0[t] = std::min(np, (int)std::lround(np * std::sqrt((double)t / T)));
+    row0[T] = np;
+    std::vector<size_t> cnt((size_t)T * NL, 0);
+    parallel_tasks(T, T, [&](int t) {
+        size_t* c = &cnt[(size_t)t * NL];
+        for (int x = row0[t]; x < row0[t + 1]; ++x)
+            for (int y = 0; y <= x; ++y) {
+                unsigned b, k;
+                const int l = list_of(x, y, b, k);
+                if (l >= 0) ++c[l];
+            }
+    });
+    // one buffer for all lists; the start of thread t's run in list l
+    std::vector<size_t> lstart(NL + 1, 0), pos((size_t)T * NL);
+    for (int l = 0; l < NL; ++l) {
+        size_t run = lstart[l];
+        for (int t = 0; t < T; ++t) {
+            pos[(size_t)t * NL + l] = run;
+            run += cnt[(size_t)t * NL + l];
         }
-    std::vector<std::vector<uint2>> raw(2 * NQCLASS);
-    for (int l = 0; l < 2 * NQCLASS; ++l) raw[l].resize(cnt[l]);
-    std::fill(cnt.begin(), cnt.end(), 0);
-    for (int x = 0; x < np; ++x)
-        for (int y = 0; y <= x; ++y) {
-            unsigned b, k;
-            const int l = list_of(x, y, b, k);
-            if (l >= 0) raw[l][cnt[l]++] = make_uint2(b, k);
-        }
+        lstart[l + 1] = run;
+    }
+    std::unique_ptr<uint2[]> raw(new uint2[std::max<size_t>(lstart[NL], 1)]);
+    parallel_tasks(T, T, [&](int t) {
+        size_t* p = &pos[(size_t)t * NL];
+        for (int x = row0[t]; x < row0[t + 1]; ++x)
+            for (int y = 0; y <= x; ++y) {
+                unsigned b, k;
+                const int l = list_of(x, y, b, k);
+                if (l >= 0) raw[p[l]++] = make_uint2(b, k);
+            }
+    });
     int maxw = 0;
     for (int x = 0; x < np; ++x) maxw = std::max(maxw, nprim[x]);
     maxw = maxw * maxw;
-    std::vector<size_t> hist((size_t)maxw + 2);
-    for (int c = 0; c < NQCLASS; ++c)
-        for (int p = 0; p < 2; ++p) {
-            const std::vector<uint2>& v = raw[2 * c + p];
-            std::vector<uint2>& out = ql.list[c][p];
-            out.resize(v.size());
-            // stable counting sort, w descending: slot of w starts after all larger w
-            std::fill(hist.begin(), hist.end(), 0);
-            for (const uint2& e : v) ++hist[(size_t)(maxw - nprim[e.x] * nprim[e.y])];
-            size_t run = 0;
-            for (size_t k = 0; k < hist.size(); ++k) {
-                const size_t h = hist[k];
-                hist[k] = run;
-                run += h;
-            }
-            for (const uint2& e : v) out[hist[(size_t)(maxw - nprim[e.x] * nprim[e.y])]++] = e;
-            for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = 0;
-            const bool fa = qc_family_a(c);
-            for (const uint2& e : out) ++ql.bucket[c][p][fa ? group_bucket(nprim[e.x] * nprim[e.y]) : NGROUP - 1];
+    parallel_tasks(NL, T, [&](int l) {
+        const int c = l / 2, p = l % 2;
+        const uint2* v = raw.get() + lstart[l];
+        const size_t n = lstart[l + 1] - lstart[l];
+        std::vector<uint2>& out = ql.list[c][p];
+        out.resize(n);
+        // stable counting sort, w descending: slot of w starts after all larger w
+        std::vector<size_t> hist((size_t)maxw + 2, 0);
+        for (size_t i = 0; i < n; ++i) ++hist[(size_t)(maxw - nprim[v[i].x] * nprim[v[i].y])];
+        size_t run = 0;
+        for (size_t k = 0; k < hist.size(); ++k) {
+            const size_t h = hist[k];
+            hist[k] = run;
+            run += h;
         }
+        for (size_t i = 0; i < n; ++i) out[hist[(size_t)(maxw - nprim[v[i].x] * nprim[v[i].y])]++] = v[i];
+        for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = 0;
+        const bool fa = qc_family_a(c);
+        for (const uint2& e : out) ++ql.bucket[c][p][fa ? group_bucket(nprim[e.x] * nprim[e.y]) : NGROUP - 1];
+    });
 }
 
 // Work units of one launch (one lane-group bucket of n quartets, sorted by
