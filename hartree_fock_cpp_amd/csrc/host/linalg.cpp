@@ -46,6 +46,7 @@ namespace {
 // Givens updates touch contiguous memory.
 // With refl != nullptr, Q is not formed: the reflector of step k (v over rows
 // k+1..n-1, scale beta[k]; beta 0 = identity) is kept in row k of refl.
+// Only the lower triangle of A is read and updated.
 void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::vector<double>& e,
                     std::vector<double>& Qc, std::vector<double>* refl = nullptr, std::vector<double>* betas = nullptr) {
     d.assign(n, 0.0);
@@ -73,39 +74,44 @@ void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::
         for (int i = 0; i < m; ++i) vv += v[i] * v[i];
         if (vv == 0.0) continue;
         const double beta = 2.0 / vv;
-        // p = beta * A' v  (A' = trailing block; four partial sums so the
-        // dot products vectorise)
+        // p = beta * A' v from the lower triangle of the trailing block A':
+        // row i gives its dot with v_0..v_i and, transposed, adds A'_ij v_i
+        // to p_j for j < i (one pass over half the block)
         const double* __restrict__ vp = v.data();
+        double* __restrict__ pp = p.data();
+        std::fill(pp, pp + m, 0.0);
         for (int i = 0; i < m; ++i) {
             const double* __restrict__ row = &A[(size_t)(k + 1 + i) * n + k + 1];
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            const double vi = vp[i];
+            double s0 = 0.0, s1 = 0.0;
             int j = 0;
-            for (; j + 4 <= m; j += 4) {
+            for (; j + 2 <= i; j += 2) {
                 s0 += row[j] * vp[j];
                 s1 += row[j + 1] * vp[j + 1];
-                s2 += row[j + 2] * vp[j + 2];
-                s3 += row[j + 3] * vp[j + 3];
+                pp[j] += row[j] * vi;
+                pp[j + 1] += row[j + 1] * vi;
             }
-            for (; j < m; ++j) s0 += row[j] * vp[j];
-            p[i] = beta * ((s0 + s1) + (s2 + s3));
+            for (; j < i; ++j) {
+                s0 += row[j] * vp[j];
+                pp[j] += row[j] * vi;
+            }
+            pp[i] += (s0 + s1) + row[i] * vi;
         }
+        for (int i = 0; i < m; ++i) pp[i] *= beta;
         double pv = 0.0;
         for (int i = 0; i < m; ++i) pv += p[i] * v[i];
         const double c = 0.5 * beta * pv;
         for (int i = 0; i < m; ++i) w[i] = p[i] - c * v[i];
-        // A' -= v w^T + w v^T
+        // A' -= v w^T + w v^T on the lower triangle (the upper one is stale
+        // from here on and never read)
         const double* __restrict__ wp = w.data();
         for (int i = 0; i < m; ++i) {
             double* __restrict__ row = &A[(size_t)(k + 1 + i) * n + k + 1];
             const double vi = vp[i], wi = wp[i];
-            for (int j = 0; j < m; ++j) row[j] -= vi * wp[j] + wi * vp[j];
+            for (int j = 0; j <= i; ++j) row[j] -= vi * wp[j] + wi * vp[j];
         }
         A[(size_t)(k + 1) * n + k] = alpha;
-        A[(size_t)k * n + k + 1] = alpha;
-        for (int i = 1; i < m; ++i) {
-            A[(size_t)(k + 1 + i) * n + k] = 0.0;
-            A[(size_t)k * n + k + 1 + i] = 0.0;
-        }
+        for (int i = 1; i < m; ++i) A[(size_t)(k + 1 + i) * n + k] = 0.0;
         if (refl) {
             std::copy(v.begin(), v.begin() + m, refl->begin() + (size_t)k * n + k + 1);
             (*betas)[k] = beta;
@@ -127,7 +133,7 @@ void tridiagonalize(int n, std::vector<double>& A, std::vector<double>& d, std::
         }
     }
     for (int i = 0; i < n; ++i) d[i] = A[(size_t)i * n + i];
-    for (int i = 0; i + 1 < n; ++i) e[i] = A[(size_t)i * n + i + 1];
+    for (int i = 0; i + 1 < n; ++i) e[i] = A[(size_t)(i + 1) * n + i];
 }
 
 // Implicit symmetric QR with Wilkinson shift on (d, e); rotations applied to
