@@ -55,24 +55,14 @@ bool class_uses_family_a(int qc) {
 // k: the k-keyed terms are summed over each run with a segmented wave scan
 // (DPP lane shifts, no LDS traffic) and added by the run's last lane, one LDS
 // atomic per run instead of 64 atomics on one address; the l-keyed terms hit
-// distinct addresses.  (k, l) come from KL arithmetically, so the gather of
-// Dj_kl issues with the integral load, and each thread keeps JK_UNROLL
-// elements in flight.
+// distinct addresses.  Each thread keeps JK_UNROLL elements in flight (their
+// pair ids, integrals and Dj_kl gathers issue together); reading (k, l) from
+// the pair table measured faster than computing it from KL with sqrtf.
 // The column terms Jacc_kl += 2f v Dj_ij are k_j_columns' (a per-element
 // atomic here would cost one global atomic per stored integral).
 constexpr int JK_BLOCK = 256;
 constexpr int JK_UNROLL = 4;
 constexpr int JK_MAXK = 900;   // 2 spins x 4 rows x K doubles must fit 64 KiB of LDS
-
-// (k, l), k >= l, of the k-major pair index p = k(k+1)/2 + l (p < 2^21, so
-// 8p + 1 is exact in float; the two corrections absorb the rounding of sqrtf).
-__device__ inline void pair_of(int p, int& k, int& l) {
-    int r = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
-    if ((r + 1) * (r + 2) / 2 <= p) ++r;
-    if (r * (r + 1) / 2 > p) --r;
-    k = r;
-    l = p - r * (r + 1) / 2;
-}
 
 // value of lane (lane - shift) under DPP control CTRL, 0 where that lane lies
 // outside the row or the destination row is masked off by RM
@@ -142,8 +132,8 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk_unique(const double* __restrict
         for (int u = 0; u < JK_UNROLL; ++u) {
             const int KL = KL0 + u * JK_BLOCK + tid;
             const bool in = KL <= IJ;
-            int k, l;
-            pair_of(in ? KL : 0, k, l);
+            const int2 kl = pairs[in ? KL : 0];
+            const int k = kl.x, l = kl.y;
             kk[u] = in ? k : 0x3fffffff;   // lanes past the row form a run of their own
             ll[u] = l;
             v[u] = in ? eri[rowbase + KL] : 0.0;
