@@ -6,7 +6,10 @@
  * status codes as int (0 = success, negative = error class; the message is
  * available from hfx_last_error()).  All functions are synchronous unless a
  * stream argument is taken.  Thread compatibility: distinct hfx_basis /
- * hfx_fock objects may be used from distinct threads concurrently.
+ * hfx_fock objects may be used from distinct threads concurrently.  Planning
+ * a large basis (above 2^18 shell-pair pairs: the first unique build or
+ * hfx_fock_create at a new threshold) runs on up to 16 short-lived host
+ * threads of the library's own, joined before the call returns.
  *
  * Conventions reproduced from the reference (all cited file:line are in
  * KLYYCYNN/hartree_fock_cpp):
