@@ -200,6 +200,58 @@ void tridiag_qr(int n, std::vector<double>& d, std::vector<double>& e, std::vect
     }
 }
 
+// Eigenvalues of the symmetric tridiagonal (d, e) by the root-free implicit
+// QR step of Pal, Walker and Kahan (B. N. Parlett, The Symmetric Eigenvalue
+// Problem, sec. 8-15; the scheme of LAPACK dsterf): the sweep carries the
+// squared off-diagonals and the squared rotation cosine / sine, so it takes
+// no square root per rotation (one per sweep, for the Wilkinson shift of the
+// trailing 2 x 2).  Eigenvalues overwrite d, unsorted; e is destroyed.
+void tridiag_eigenvalues(int n, std::vector<double>& d, std::vector<double>& e) {
+    const double eps = 2.220446049250313e-16;
+    const double eps2 = eps * eps;
+    std::vector<double>& e2 = e;
+    for (int i = 0; i + 1 < n; ++i) e2[i] *= e2[i];
+    int iter = 0;
+    const int max_iter = 60 * std::max(n, 1);
+    int q = n - 1;
+    while (q > 0 && iter < max_iter) {
+        // deflate: |e_i| <= eps (|d_i| + |d_i+1|), squared
+        for (int i = 0; i < q; ++i) {
+            const double t = std::fabs(d[i]) + std::fabs(d[i + 1]);
+            if (e2[i] <= eps2 * t * t) e2[i] = 0.0;
+        }
+        while (q > 0 && e2[q - 1] == 0.0) --q;
+        if (q == 0) break;
+        int p = q - 1;
+        while (p > 0 && e2[p - 1] != 0.0) --p;
+        // Wilkinson shift from the trailing 2x2 of block [p..q]
+        const double a = d[q - 1], c = d[q], b2 = e2[q - 1];
+        const double delta = 0.5 * (a - c);
+        const double sgn = delta >= 0 ? 1.0 : -1.0;
+        const double sigma = c - b2 / (delta + sgn * std::hypot(delta, std::sqrt(b2)));
+        // e2[k] != 0 inside the block, so R > 0 below
+        double cs = 1.0, sn = 0.0;
+        double gamma = d[p] - sigma;
+        double P = gamma * gamma;
+        for (int k = p; k < q; ++k) {
+            const double bb = e2[k];
+            const double R = P + bb;
+            if (k > p) e2[k - 1] = sn * R;
+            const double oldc = cs;
+            cs = P / R;
+            sn = bb / R;
+            const double oldgam = gamma;
+            const double alpha = d[k + 1];
+            gamma = cs * (alpha - sigma) - sn * oldgam;
+            d[k] = oldgam + (alpha - gamma);
+            P = cs != 0.0 ? gamma * gamma / cs : oldc * bb;
+        }
+        e2[q - 1] = sn * P;
+        d[q] = gamma + sigma;
+        ++iter;
+    }
+}
+
 }  // namespace
 
 void sym_eig(int n, const double* A, double* w, double* V) {
@@ -299,9 +351,9 @@ void sym_eig_lowest(int n, const double* A, int m, double* w, double* V) {
             a[(size_t)j * n + i] = s;
         }
     tridiagonalize(n, a, d, e, Qc, &refl, &betas);
-    // eigenvalues: QR without vectors on a copy of T
+    // eigenvalues: root-free QR on a copy of T
     std::vector<double> dq(d), eq(e);
-    tridiag_qr(n, dq, eq, Qc, false);
+    tridiag_eigenvalues(n, dq, eq);
     std::sort(dq.begin(), dq.end());
     for (int k = 0; k < n; ++k) w[k] = dq[k];
     // eigenvectors of T for the m lowest eigenvalues by inverse iteration,
