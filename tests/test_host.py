@@ -130,3 +130,36 @@ def test_host_eigensolver_lowest_clusters(kind):
         assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * max(1, np.abs(w).max())
         assert np.abs(Vm.T @ Vm - np.eye(m)).max() < 1e-12
         assert np.abs(A @ Vm - Vm * w[:m]).max() < 1e-12 * max(1, np.abs(w).max())
+
+
+@pytest.mark.parametrize("kind", ["tridiagonal", "nearly_split", "split", "graded", "zero_diagonal"])
+def test_host_eigenvalues_root_free_qr(kind):
+    """The eigenvalue sweep (root-free QR on the tridiagonal form) on inputs
+    that stress its deflation: already-tridiagonal matrices, off-diagonals at
+    1e-9 of the diagonal, exact zeros (split blocks), magnitudes spread over
+    six decades, and a zero diagonal (deflation next to exact zeros).
+    Eigenvalues as numpy's LAPACK eigvalsh."""
+    rng = np.random.default_rng(11)
+    n = 60
+    d = rng.standard_normal(n)
+    e = rng.standard_normal(n - 1)
+    if kind == "nearly_split":
+        e *= 1e-9
+    elif kind == "split":
+        e[::4] = 0.0
+    elif kind == "graded":
+        d *= np.logspace(0, 6, n)
+        e *= np.logspace(0, 6, n - 1) * 0.1
+    elif kind == "zero_diagonal":
+        d[:] = 0.0
+    A = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    if kind == "tridiagonal":
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        A = Q @ A @ Q.T
+        A = 0.5 * (A + A.T)
+    w, V = _lowest(A, 5)
+    wr = np.linalg.eigvalsh(A)
+    s = max(1.0, np.abs(wr).max())
+    assert np.abs(w - wr).max() < 1e-12 * s
+    Vm = V[:, :5]
+    assert np.abs(A @ Vm - Vm * w[:5]).max() < 1e-11 * s
