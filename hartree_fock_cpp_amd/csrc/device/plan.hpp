@@ -253,9 +253,10 @@ inline void parallel_tasks(int ntask, int nthreads, F&& fn) {
 // n-octane/cc-pVDZ at 1e-12), with no per-element reallocation.  Large
 // triangles are cut into row ranges of equal area, one per thread; a thread's
 // entries of a list follow those of the threads before it, so the lists come
-// out in the same order as from one thread.
+// out in the same order as from one thread.  threads > 0 overrides the
+// automatic thread count (tests compare thread counts).
 inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, double thr, bool mixed, double split,
-                          QuartetLists& ql) {
+                          QuartetLists& ql, int threads = 0) {
     constexpr int NL = 2 * NQCLASS;
     const int np = bp.npair;
     std::vector<int> nprim(np), pcls(np);
@@ -274,7 +275,7 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
         ket = (unsigned)k;
         return 2 * quartet_class(pcls[b], pcls[k]) + ((mixed && qq < split) ? 1 : 0);
     };
-    const int T = plan_threads((size_t)np * (np + 1) / 2);
+    const int T = threads > 0 ? threads : plan_threads((size_t)np * (np + 1) / 2);
     std::vector<int> row0(T + 1);   // thread t takes rows [row0[t], row0[t+1])
     for (int t = 0; t <= T; ++t) row0[t] = std::min(np, (int)std::lround(np * std::sqrt((double)t / T)));
     row0[T] = np;

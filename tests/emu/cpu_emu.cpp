@@ -242,6 +242,30 @@ int emu_class_counts(EMU_BASIS_ARGS, double threshold, int mixed, double split, 
     return 0;
 }
 
+// Plans the screened quartets on one thread and on `threads` threads and
+// compares the lists and lane-group buckets entry by entry; returns the number
+// of (class, precision) lists that differ (0: identical) and the total quartet
+// count in *nq.
+int emu_plan_threads_agree(EMU_BASIS_ARGS, double threshold, int mixed, double split, int threads, long long* nq) {
+    EmuCtx c;
+    if (make_ctx(K, xyz, lmn, poff, alpha, coeff, norm, c)) return -1;
+    QuartetLists one, many;
+    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, one, 1);
+    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, many, threads);
+    int bad = 0;
+    *nq = (long long)one.total();
+    for (int q = 0; q < NQCLASS; ++q)
+        for (int p = 0; p < 2; ++p) {
+            const auto& a = one.list[q][p];
+            const auto& b = many.list[q][p];
+            bool same = a.size() == b.size();
+            for (size_t i = 0; same && i < a.size(); ++i) same = a[i].x == b[i].x && a[i].y == b[i].y;
+            for (int g = 0; same && g < NGROUP; ++g) same = one.bucket[q][p][g] == many.bucket[q][p][g];
+            bad += same ? 0 : 1;
+        }
+    return bad;
+}
+
 // Boys values from the product's table path (FP64 and FP32).
 void emu_boys(int mmax, double T, double* F64, float* F32) {
     static std::vector<double> tab;

@@ -172,3 +172,20 @@ def test_emu_primitive_slices_are_exact(S):
     assert np.abs(JS - J1).max() < 1e-12 and np.abs(KS - K1).max() < 1e-12
     exact = orc.eri_unique(d, thr, exact=True)
     assert np.abs(uS - exact).max() < 1e-12
+
+
+@pytest.mark.parametrize("threads,mixed", [(2, 0), (7, 0), (16, 1)])
+def test_plan_threads_agree(threads, mixed):
+    """The quartet planner cut over several threads (row ranges of equal
+    area, per-list counting sorts in parallel) yields the same lists, in the
+    same order, with the same lane-group buckets as on one thread: benzene /
+    cc-pVDZ (1.1 M quartets), FP64 and the mixed split."""
+    from hartree_fock_cpp_amd import workloads as W
+    from hartree_fock_cpp_amd.basis import basis_construction
+    d = basis_construction(W.benzene(), "cc-pVDZ")
+    K, keep, a = _bargs(d)
+    nq = C.c_longlong(0)
+    bad = emu.emu_plan_threads_agree(*a, C.c_double(1e-8), C.c_int(mixed), C.c_double(1e-5), C.c_int(threads),
+                                     C.byref(nq))
+    assert bad == 0, (bad, emu.emu_last_error())
+    assert nq.value > 1_000_000
