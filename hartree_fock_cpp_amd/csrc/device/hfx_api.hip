@@ -11,6 +11,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -403,11 +404,19 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target
 // for the FP64 unique build).
 int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets target, DevLists& dl) {
     dfree(dl.d);
-    dl.d = nullptr;
     dl.valid = false;
-    QuartetLists ql;
-    plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql);
-    if (int rc = upload_lists(b, ql, target, dl)) return rc;
+    // host work in the C ABI's call: failures come back as status codes
+    try {
+        QuartetLists ql;
+        plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql);
+        if (int rc = upload_lists(b, ql, target, dl)) return rc;
+    } catch (const std::bad_alloc&) {
+        dfree(dl.d);
+        return fail(HFX_ERR_NOMEM, "host memory for the quartet plan");
+    } catch (const std::exception& e) {
+        dfree(dl.d);
+        return fail(HFX_ERR_DEVICE, std::string("quartet plan: ") + e.what());
+    }
     dl.threshold = thr;
     dl.mixed = mixed;
     dl.split = split;
@@ -507,7 +516,13 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
     if (int rc = ensure_consts()) return rc;
     hfx_basis* b = new hfx_basis();
     std::string err;
-    int rc = build_plan(K, xyz, lmn, poff, alpha, coeff, norm, b->plan, err);
+    int rc = 0;
+    try {
+        rc = build_plan(K, xyz, lmn, poff, alpha, coeff, norm, b->plan, err);
+    } catch (const std::bad_alloc&) {
+        delete b;
+        return fail(HFX_ERR_NOMEM, "host memory for the basis tables");
+    }
     if (rc) {
         delete b;
         return fail(rc == -4 ? HFX_ERR_UNSUPPORTED : HFX_ERR_INVALID, err);

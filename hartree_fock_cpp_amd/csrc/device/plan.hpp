@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <exception>
 #include <memory>
 #include <string>
 #include <thread>
@@ -216,8 +217,10 @@ inline int plan_threads(size_t work) {
 }
 
 // fn(task) for every task in [0, ntask), spread over up to nthreads threads
-// (the caller's thread included) by a shared counter.  fn must not throw.
-// If a thread cannot be started, the threads already running take its tasks.
+// (the caller's thread included) by a shared counter.  If a thread cannot be
+// started, the threads already running take its tasks.  The first exception
+// a task throws stops the hand-out of further tasks and is rethrown on the
+// caller's thread once all threads have joined.
 template <class F>
 inline void parallel_tasks(int ntask, int nthreads, F&& fn) {
     if (nthreads <= 1 || ntask <= 1) {
@@ -225,8 +228,18 @@ inline void parallel_tasks(int ntask, int nthreads, F&& fn) {
         return;
     }
     std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr err;
     auto worker = [&] {
-        for (int t; (t = next.fetch_add(1)) < ntask;) fn(t);
+        for (int t; (t = next.fetch_add(1)) < ntask;) {
+            try {
+                fn(t);
+            } catch (...) {
+                if (!failed.exchange(true)) err = std::current_exception();
+                next.store(ntask);
+                return;
+            }
+        }
     };
     std::vector<std::thread> th;
     const int n = std::min(nthreads, ntask);
@@ -239,6 +252,7 @@ inline void parallel_tasks(int ntask, int nthreads, F&& fn) {
     }
     worker();
     for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
 }
 
 // Screened canonical shell quartets (x >= y over canonical pair ids), split by

@@ -7,6 +7,7 @@
 // product library, smoke() or bench.py.
 #include <cstring>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -264,6 +265,25 @@ int emu_plan_threads_agree(EMU_BASIS_ARGS, double threshold, int mixed, double s
             bad += same ? 0 : 1;
         }
     return bad;
+}
+
+// Runs parallel_tasks over ntask tasks on `threads` threads; task throw_at
+// (if in range) throws.  Returns 1 when the exception reached the caller, 0
+// when none was thrown, -1 when one was expected and lost; *ran counts the
+// tasks that completed.
+int emu_parallel_tasks_check(int ntask, int threads, int throw_at, int* ran) {
+    std::atomic<int> n{0};
+    try {
+        parallel_tasks(ntask, threads, [&](int t) {
+            if (t == throw_at) throw std::runtime_error("task failed");
+            n.fetch_add(1);
+        });
+    } catch (const std::runtime_error& e) {
+        *ran = n.load();
+        return std::string(e.what()) == "task failed" ? 1 : -1;
+    }
+    *ran = n.load();
+    return (throw_at >= 0 && throw_at < ntask) ? -1 : 0;
 }
 
 // Boys values from the product's table path (FP64 and FP32).

@@ -189,3 +189,15 @@ def test_plan_threads_agree(threads, mixed):
                                      C.byref(nq))
     assert bad == 0, (bad, emu.emu_last_error())
     assert nq.value > 1_000_000
+
+
+@pytest.mark.parametrize("threads", [1, 4, 16])
+def test_parallel_tasks_propagates_exceptions(threads):
+    """The planner's thread helper runs every task once when none throws, and
+    hands a task's exception back to the calling thread (not std::terminate)
+    when one does."""
+    ran = C.c_int(0)
+    assert emu.emu_parallel_tasks_check(C.c_int(200), C.c_int(threads), C.c_int(-1), C.byref(ran)) == 0
+    assert ran.value == 200
+    assert emu.emu_parallel_tasks_check(C.c_int(200), C.c_int(threads), C.c_int(57), C.byref(ran)) == 1
+    assert ran.value < 200
