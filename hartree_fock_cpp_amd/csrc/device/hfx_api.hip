@@ -923,6 +923,9 @@ namespace {
 // Re-plans a direct context's work units for slice target t; the previous
 // plan's builds must have completed (its lists go back to the cache).
 int replan(hfx_fock* f, long long t) {
+    // the lists about to be replaced may still be read by the last build
+    // (ev_done is recorded after every build, on its stream)
+    if (f->ev_done) HIPCHK(hipEventSynchronize(f->ev_done));
     if (f->graph) {
         HIPCHK(hipGraphExecDestroy(f->graph));
         f->graph = nullptr;
