@@ -408,13 +408,43 @@ void sym_eig_lowest(int n, const double* A, int m, double* w, double* V) {
         }
         std::copy(x.begin(), x.end(), Z.begin() + (size_t)k * n);
     }
-    // back to A's basis: q = H_0 H_1 ... H_{n-3} z
-    for (int k = 0; k < m; ++k) {
-        double* z = &Z[(size_t)k * n];
+    // back to A's basis: q = H_0 H_1 ... H_{n-3} z, four vectors per pass
+    // over each reflector (four independent dot products in flight)
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+        double* __restrict__ z0 = &Z[(size_t)k * n];
+        double* __restrict__ z1 = z0 + n;
+        double* __restrict__ z2 = z1 + n;
+        double* __restrict__ z3 = z2 + n;
         for (int j = n - 3; j >= 0; --j) {
             const double beta = betas[j];
             if (beta == 0.0) continue;
-            const double* v = &refl[(size_t)j * n];
+            const double* __restrict__ v = &refl[(size_t)j * n];
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            for (int i = j + 1; i < n; ++i) {
+                s0 += v[i] * z0[i];
+                s1 += v[i] * z1[i];
+                s2 += v[i] * z2[i];
+                s3 += v[i] * z3[i];
+            }
+            s0 *= beta;
+            s1 *= beta;
+            s2 *= beta;
+            s3 *= beta;
+            for (int i = j + 1; i < n; ++i) {
+                z0[i] -= s0 * v[i];
+                z1[i] -= s1 * v[i];
+                z2[i] -= s2 * v[i];
+                z3[i] -= s3 * v[i];
+            }
+        }
+    }
+    for (; k < m; ++k) {
+        double* __restrict__ z = &Z[(size_t)k * n];
+        for (int j = n - 3; j >= 0; --j) {
+            const double beta = betas[j];
+            if (beta == 0.0) continue;
+            const double* __restrict__ v = &refl[(size_t)j * n];
             double s = 0.0;
             for (int i = j + 1; i < n; ++i) s += v[i] * z[i];
             s *= beta;
