@@ -609,17 +609,70 @@ int hfx_eri_unique_device(hfx_basis* b, double threshold, double* d_out, size_t 
     return eri_unique_into(b, threshold, d_out, stream ? (hipStream_t)stream : b->stream);
 }
 
+namespace {
+
+// Device-to-host copy of a large array into the caller's (pageable) memory:
+// the DMA fills two pinned chunks in turn while host threads move the chunk
+// that landed before into place, so the copy engine and the host copies
+// overlap.  A chunk is refilled only after its host copy has returned.
+// Arrays up to one chunk take a single plain copy.
+int copy_to_host(double* out, const double* d, size_t n, hipStream_t st) {
+    constexpr size_t CH = size_t(32) << 20;
+    const size_t bytes = n * sizeof(double);
+    if (bytes <= CH) {
+        HIPCHK(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    void* h = nullptr;
+    HIPCHK(hfx_detail::host_alloc(&h, 2 * CH));
+    char* stage[2] = {static_cast<char*>(h), static_cast<char*>(h) + CH};
+    hipEvent_t ev[2] = {};
+    hipError_t e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    const size_t nch = (bytes + CH - 1) / CH;
+    auto len = [&](size_t c) { return std::min(CH, bytes - c * CH); };
+    const char* src = reinterpret_cast<const char*>(d);
+    char* dst = reinterpret_cast<char*>(out);
+    constexpr int NCOPY = 8;   // host threads per chunk copy
+    for (size_t c = 0; c <= nch && e == hipSuccess; ++c) {
+        if (c < nch) {
+            e = hipMemcpyAsync(stage[c % 2], src + c * CH, len(c), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipEventRecord(ev[c % 2], st);
+        }
+        if (c >= 1 && e == hipSuccess) {
+            const size_t p = c - 1, lp = len(p);
+            e = hipEventSynchronize(ev[p % 2]);
+            if (e != hipSuccess) break;
+            const size_t slice = (lp + NCOPY - 1) / NCOPY;
+            try {
+                parallel_tasks(NCOPY, NCOPY, [&](int t) {
+                    const size_t o = (size_t)t * slice;
+                    if (o < lp) std::memcpy(dst + p * CH + o, stage[p % 2] + o, std::min(slice, lp - o));
+                });
+            } catch (...) {
+                for (size_t o = 0; o < lp; o += slice)   // no threads: copy on this one
+                    std::memcpy(dst + p * CH + o, stage[p % 2] + o, std::min(slice, lp - o));
+            }
+        }
+    }
+    if (e != hipSuccess) (void)hipStreamSynchronize(st);   // no copy may still target the staging
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    hfx_detail::host_free(h);
+    if (e != hipSuccess) return fail(HFX_ERR_DEVICE, std::string("D2H: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
 int hfx_eri_unique(hfx_basis* b, double threshold, double* out, size_t Q) {
     if (!b || !out) return fail(HFX_ERR_INVALID, "null argument");
     if (Q != n_unique(b->plan.K)) return fail(HFX_ERR_INVALID, "Q must equal P(P+1)/2 with P = K(K+1)/2");
     double* d = nullptr;
     if (int rc = dalloc(&d, Q)) return rc;
     int rc = eri_unique_into(b, threshold, d, b->stream);
-    if (rc == 0) {
-        hipError_t e = hipMemcpyAsync(out, d, Q * sizeof(double), hipMemcpyDeviceToHost, b->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-        if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("D2H: ") + hipGetErrorString(e));
-    }
+    if (rc == 0) rc = copy_to_host(out, d, Q, b->stream);
     dfree(d);
     return rc;
 }

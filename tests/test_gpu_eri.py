@@ -88,3 +88,31 @@ def test_class_timing_keeps_values(gpu, mode):
     print(f"mode {mode}: {int((ms > 0).sum())} timed lists, {int((nq > 0).sum())} launched, sum {ms.sum():.3f} ms")
     assert np.all(ms[nq > 0] > 0) and np.all(ms[nq == 0] == 0)
     assert np.abs(got - ref).max() < 1e-14
+
+
+def test_eri_unique_host_copy_matches_device_array(gpu):
+    """The host-buffer entry point (the drop-in's calling form) returns the
+    device-resident array bit for bit when the copy runs in pinned chunks:
+    benzene / cc-pVDZ, Q = 26,357,430 (211 MB, seven chunks, the last partial)."""
+    import ctypes as C
+
+    import torch
+
+    from hartree_fock_cpp_amd import _native as N
+    from hartree_fock_cpp_amd import workloads as W
+    from hartree_fock_cpp_amd.basis import basis_construction
+
+    b = gpu.Basis.from_arrays(basis_construction(W.benzene(), "cc-pVDZ"))
+    Q = b.n_quartets
+    print(f"benzene/cc-pVDZ Q={Q}", flush=True)
+    dev = torch.empty(Q, dtype=torch.float64, device="cuda")
+    st = torch.cuda.Stream()
+    N.check(N.lib.hfx_eri_unique_device(b.handle(), C.c_double(1e-8), C.c_void_p(dev.data_ptr()), C.c_size_t(Q),
+                                        C.c_void_p(st.cuda_stream)))
+    torch.cuda.synchronize()
+    ref = dev.cpu().numpy()
+    del dev
+    got = gpu.ERI_GPU(b, 1e-8)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+    assert np.count_nonzero(got) > Q // 2
