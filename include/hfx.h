@@ -9,7 +9,8 @@
  * hfx_fock objects may be used from distinct threads concurrently.  Planning
  * a large basis (above 2^18 shell-pair pairs: the first unique build or
  * hfx_fock_create at a new threshold) runs on up to 16 short-lived host
- * threads of the library's own, joined before the call returns.
+ * threads of the library's own, joined before the call returns; so does the
+ * host copy of hfx_eri_unique above 32 MB (up to 8 threads).
  *
  * Conventions reproduced from the reference (all cited file:line are in
  * KLYYCYNN/hartree_fock_cpp):
