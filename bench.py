@@ -100,7 +100,7 @@ def cpu_baseline(arrs, bounds, thr, budget_s, seed=0):
                      f"workload, {t:.1f} s of single-core basis_repulsion (include/integrals.hpp:701-729)"}
     # the same sample split over every host core (ctypes releases the GIL in the call)
     import threading
-    # the host-core share of one GPU (OMP_NUM_THREADS on the GPU boxes), at most 16
+    # the host cores that go with one GPU (OMP_NUM_THREADS where it is set), at most 16
     cores = max(1, min(16, os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
     parts = np.array_split(keep[64:64 + n], cores)
     t0 = time.perf_counter()
@@ -318,8 +318,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build runs on a single GPU; multi-GPU sharding is described "
-              "in DESIGN.md and not launched.", file=sys.stderr)
+        print("bench.py measures one MI355X: the engine runs on a single GPU; sharding over several GPUs "
+              "(work-unit lists split, J/K all-reduced) is designed but not built.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
