@@ -208,9 +208,14 @@ def stored_jk(hfx, N, basis, thr, steps):
     hbm = 2 * 8 * Q   # the unique array read by both kernels; K^2 matrices are negligible
     gbs = hbm / (ms * 1e-3) / 1e9
     log(f"  stored J/K: {ms:.3f} ms per build, {gbs:.0f} GB/s")
+    traffic = None   # measured HBM bytes per build (FETCH_SIZE / WRITE_SIZE runs, profiles/)
+    tpath = os.path.join(ROOT, "profiles", "traffic_benzene_stored_jk.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_build")
     return {"workload": "benzene RHF/cc-pVDZ J/K from the stored unique array (FP64)", "ms_per_build": ms,
             "hbm_bytes": hbm, "achieved_GBps": gbs, "peak_GBps": PEAK_HBM_GBS, "frac": gbs / PEAK_HBM_GBS,
-            "timing": f"HIP events on the launch stream over {steps} builds"}
+            "traffic": traffic, "timing": f"HIP events on the launch stream over {steps} builds"}
 
 
 def pcie_inclusive(hfx, N, basis, thr, q_screened, reps=5):
