@@ -83,35 +83,42 @@ def eri_at(L, K, b, bd, thr, qids):
 
 def sample_qids(K, b, per_class, seed, thr_bd=None):
     """Seeded sample of unique quartet ids, stratified by the angular class of
-    the function quartet (sorted la>=lb, lc>=ld, bra>=ket)."""
+    the function quartet (sorted la>=lb, lc>=ld, bra>=ket): candidates are
+    drawn uniformly from the Q unique ids in batches and each class keeps the
+    first per_class distinct ids it receives."""
     rng = np.random.default_rng(seed)
-    lsum = b["lmn"].reshape(K, 3).sum(1)
-    idx = {}
+    lsum = b["lmn"].reshape(K, 3).sum(1).astype(np.int64)
     P = K * (K + 1) // 2
-    pi = np.array([i for i in range(K) for j in range(i + 1)])
-    pj = np.array([j for i in range(K) for j in range(i + 1)])
-    # draw candidates at random and bin them by class until each bin is full
     Q = P * (P + 1) // 2
-    need = per_class
-    tries = 0
-    while tries < 400:
-        tries += 1
+    pi = np.repeat(np.arange(K), np.arange(1, K + 1))
+    pj = np.concatenate([np.arange(i + 1) for i in range(K)])
+    lo_of_pair = np.minimum(lsum[pi], lsum[pj])
+    hi_of_pair = np.maximum(lsum[pi], lsum[pj])
+    pcode = hi_of_pair * 4 + lo_of_pair      # (la, lb) with la >= lb, ordered like the tuple
+    idx = {}
+    for tries in range(400):
         cand = rng.integers(0, Q, size=200000, dtype=np.int64)
         IJ = ((np.sqrt(8.0 * cand + 1.0) - 1.0) // 2).astype(np.int64)
         IJ -= (IJ * (IJ + 1) // 2 > cand)
         IJ += ((IJ + 1) * (IJ + 2) // 2 <= cand)
         KL = cand - IJ * (IJ + 1) // 2
-        la, lb, lc, ld = lsum[pi[IJ]], lsum[pj[IJ]], lsum[pi[KL]], lsum[pj[KL]]
-        for n in range(len(cand)):
-            a, bb = sorted((la[n], lb[n]), reverse=True)
-            c, d = sorted((lc[n], ld[n]), reverse=True)
-            key = max((a, bb), (c, d)) + min((a, bb), (c, d))
-            lst = idx.setdefault(key, [])
-            if len(lst) < need:
-                lst.append(int(cand[n]))
-        if len(idx) >= 1 and all(len(v) >= need for v in idx.values()) and tries > 20:
+        cb, ck = pcode[IJ], pcode[KL]
+        code = np.maximum(cb, ck) * 16 + np.minimum(cb, ck)
+        for c in np.unique(code):
+            have = idx.setdefault(int(c), [])
+            if len(have) >= per_class:
+                continue
+            seen = set(have)
+            for q in cand[code == c]:
+                q = int(q)
+                if q not in seen:
+                    have.append(q)
+                    seen.add(q)
+                    if len(have) >= per_class:
+                        break
+        if tries >= 20 and all(len(v) >= per_class for v in idx.values()):
             break
-    out = sorted(set(q for v in idx.values() for q in v))
+    out = sorted(q for v in idx.values() for q in v)
     return np.array(out, np.uint64)
 
 
@@ -205,9 +212,73 @@ def benzene():
     return at
 
 
+def eri_unique_threaded(fn, Q, nthreads=8, chunk=1 << 20):
+    """The full unique array through fn(qids) -> values, on host threads
+    (ctypes releases the GIL inside the C call)."""
+    from concurrent.futures import ThreadPoolExecutor
+    out = np.zeros(Q)
+
+    def work(lo):
+        hi = min(Q, lo + chunk)
+        out[lo:hi] = fn(np.arange(lo, hi, dtype=np.uint64))
+
+    with ThreadPoolExecutor(nthreads) as ex:
+        list(ex.map(work, range(0, Q, chunk)))
+    return out
+
+
+def benzene_energy_fixture(L):
+    """Benzene / cc-pVDZ RHF total energy (threshold 1e-8, epsilon 1e-10) from
+    the numpy SCF oracle on (a) the reference's own unique ERIs (libref, the
+    reference's Boys function) and (b) the oracle's exact-Boys unique ERIs,
+    with the reference's S, T, V."""
+    import time
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import scf_oracle
+    from tests import oracle_py as orc
+    from hartree_fock_cpp_amd.workloads import nuclear_repulsion
+    atoms = benzene()
+    K, b = build_basis(L, atoms, "cc-pVDZ")
+    S, T, V = one_electron(L, K, b, atoms)
+    bd = bounds_of(L, K, b)
+    thr = 1e-8
+    P = K * (K + 1) // 2
+    Q = P * (P + 1) // 2
+    d = dict(b, bounds=bd)
+    res = {"molecule": "benzene", "basis_set": "cc-pVDZ", "K": K, "threshold": thr, "epsilon": 1e-10,
+           "geometry": [[s, list(x)] for s, x in atoms]}
+    E_nuc = nuclear_repulsion(atoms)
+    ne = 42
+    t0 = time.time()
+    eri_ref = eri_unique_threaded(lambda q: eri_at(L, K, b, bd, thr, q), Q)
+    print("reference unique array", time.time() - t0, "s", flush=True)
+    E, Pm, Cm, e, it, ok = scf_oracle.rhf(S, T + V, E_nuc, ne, lambda Pd: orc.repulsion_matrix(Pd, eri_ref), eps=1e-10)
+    assert ok
+    res["E_reference_eris"] = repr(E)
+    res["iterations_reference_eris"] = it
+    del eri_ref
+    bd_x = orc.schwarz_bounds_exact(d)
+    t0 = time.time()
+    eri_x = eri_unique_threaded(lambda q: orc.eri_quartets(d, thr, q, bounds=bd_x, exact=True), Q)
+    print("exact-Boys unique array", time.time() - t0, "s", flush=True)
+    E2, Pm2, Cm2, e2, it2, ok2 = scf_oracle.rhf(S, T + V, E_nuc, ne, lambda Pd: orc.repulsion_matrix(Pd, eri_x), eps=1e-10)
+    assert ok2
+    res["E_exact_eris"] = repr(E2)
+    res["iterations_exact_eris"] = it2
+    res["note"] = ("E_reference_eris: numpy SCF oracle on the reference's own unique ERIs and S/T/V "
+                   "(include/integrals.hpp compiled in place); E_exact_eris: the same on the oracle's "
+                   "exact-Boys unique ERIs and bounds")
+    path = os.path.join(HERE, "benzene_ccpvdz_energy.json")
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+    print("wrote", path, res)
+
+
 if __name__ == "__main__":
     L = lib()
-    which = set(sys.argv[1:]) or {"boys", "vr", "h2o_sto3g", "h2o_ccpvdz", "h2_sto3g", "o2_ccpvtz", "benzene"}
+    which = set(sys.argv[1:]) or {"boys", "vr", "h2o_sto3g", "h2o_ccpvdz", "h2_sto3g", "o2_ccpvtz", "benzene",
+                                  "octane", "water4"}
     if "boys" in which:
         boys_fixture(L)
     if "vr" in which:
@@ -219,6 +290,17 @@ if __name__ == "__main__":
     if "h2o_ccpvdz" in which:
         molecule_fixture(L, "h2o_ccpvdz.npz", H2O_C1, "cc-pVDZ")
     if "o2_ccpvtz" in which:
-        molecule_fixture(L, "o2_ccpvtz.npz", O2, "cc-pVTZ", full=False, per_class=24, seed=7)
+        molecule_fixture(L, "o2_ccpvtz.npz", O2, "cc-pVTZ", full=False, per_class=200, seed=7)
     if "benzene" in which:
-        molecule_fixture(L, "benzene_ccpvdz.npz", benzene(), "cc-pVDZ", full=False, per_class=40, seed=11)
+        molecule_fixture(L, "benzene_ccpvdz.npz", benzene(), "cc-pVDZ", full=False, per_class=200, seed=11)
+    if "octane" in which or "water4" in which:
+        sys.path.insert(0, ROOT)
+        from hartree_fock_cpp_amd import workloads as W
+        if "octane" in which:   # n-octane / cc-pVDZ, jitter seed 1, Schwarz 1e-12
+            molecule_fixture(L, "octane_ccpvdz.npz", W.n_octane(seed=1), "cc-pVDZ", thr=1e-12, full=False,
+                             per_class=200, seed=1)
+        if "water4" in which:   # (H2O)4 / cc-pVTZ, jitter seed 4, Schwarz 1e-12
+            molecule_fixture(L, "water4_ccpvtz.npz", W.water_tetramer(seed=4), "cc-pVTZ", thr=1e-12, full=False,
+                             per_class=200, seed=4)
+    if "benzene_energy" in which:
+        benzene_energy_fixture(L)

@@ -56,7 +56,7 @@ def test_oracle_eri_matches_golden(name):
     assert np.array_equal(got, d["eri"]) or np.abs(got - d["eri"]).max() < 1e-15
 
 
-@pytest.mark.parametrize("name", ["o2_ccpvtz.npz", "benzene_ccpvdz.npz"])
+@pytest.mark.parametrize("name", ["o2_ccpvtz.npz", "benzene_ccpvdz.npz", "octane_ccpvdz.npz", "water4_ccpvtz.npz"])
 def test_oracle_eri_sampled_matches_golden(name):
     d = golden(name)
     got = orc.eri_quartets(d, float(d["threshold"]), d["qids"])

@@ -49,3 +49,142 @@ def test_oracle_scf_reproduces_reference_curves(row):
 def test_oracle_scf_szabo_ostlund_h2():
     E = oracle_energy([("H", (0.0, 0.0, 0.0)), ("H", (0.0, 0.0, 1.4))], "STO-3G", 0)
     assert abs(E - (-1.1167)) < 1e-4
+
+
+# ---------------------------------------------------------------- UHF pins
+# scf_oracle.uhf (include/scf.hpp:423-682, energy at scf.hpp:170-171) is pinned
+# by three facts that do not restate its own formulas:
+#  (1) with n_alpha = n_beta it reaches the RHF solution, whose energy the
+#      reference's own H2 curve and the RHF oracle fix;
+#  (2) a one-electron system has no two-electron energy (J and K cancel for
+#      the single occupied spin orbital), so E is the lowest generalized
+#      eigenvalue of (H_core, S);
+#  (3) for an open shell, its converged densities are a stationary point of
+#      the textbook UHF energy functional (Szabo & Ostlund eq. 3.353)
+#      E = sum (Pa+Pb) H + 1/2 sum_ijkl (ij|kl) [Pt_ij Pt_kl - Pa_ik Pa_jl - Pb_ik Pb_jl] + E_nuc,
+#      evaluated here by einsum on the expanded tensor: the energy equals the
+#      functional and occupied-virtual rotations change it only to second order.
+
+def _ints(atoms, bset, thr=1e-8):
+    d = basis_construction(atoms, bset)
+    Sm, T, V = orc.one_electron(d)
+    bd = orc.schwarz_bounds_exact(d)
+    eri = orc.eri_unique(d, thr, bounds=bd, exact=True)
+    return d, Sm, T + V, eri
+
+
+def _uhf(atoms, bset, na, nb, swap_k=False):
+    d, Sm, H, eri = _ints(atoms, bset)
+
+    def jk(Pa, Pb):
+        J = orc.coulomb_matrix(Pa, Pb, eri)
+        Ka, Kb = orc.exchange_matrix(Pa, eri), orc.exchange_matrix(Pb, eri)
+        return (J, Kb, Ka) if swap_k else (J, Ka, Kb)
+
+    E, Pa, Pb, it, ok = scf_oracle.uhf(Sm, H, nuclear_repulsion(atoms), na, nb, jk, eps=1e-10)
+    return E, Pa, Pb, ok, (d, Sm, H, eri)
+
+
+def _functional(Pa, Pb, H, g, E_nuc):
+    Pt = Pa + Pb
+    return (np.sum(Pt * H) + 0.5 * np.einsum("ijkl,ij,kl->", g, Pt, Pt)
+            - 0.5 * np.einsum("ijkl,ik,jl->", g, Pa, Pa) - 0.5 * np.einsum("ijkl,ik,jl->", g, Pb, Pb) + E_nuc)
+
+
+@pytest.mark.parametrize("row", [r for r in _rows() if r["source"] == "H2_vr.csv"][:4],
+                         ids=lambda r: f"H2@{float(r['R']):.4f}")
+def test_uhf_closed_shell_equals_reference_h2_curve(row):
+    R = float(row["R"])
+    atoms = [("H", (0.0, 0.0, 0.0)), ("H", (0.0, 0.0, R))]
+    E, Pa, Pb, ok, _ = _uhf(atoms, "STO-3G", 1, 1)
+    assert ok
+    assert abs(E - float(row["E"])) < 1e-9
+    assert np.abs(Pa - Pb).max() < 1e-10
+
+
+def test_uhf_closed_shell_equals_rhf_h2o():
+    atoms = [("O", (0.0, 0.0, 0.0)), ("H", (1.430, 1.107, 0.0)), ("H", (-1.430, 1.107, 0.0))]
+    E_u, Pa, Pb, ok, _ = _uhf(atoms, "STO-3G", 5, 5)
+    E_r = oracle_energy(atoms, "STO-3G", 0)
+    assert ok
+    assert abs(E_u - E_r) < 1e-10
+
+
+@pytest.mark.parametrize("bset", ["STO-3G", "cc-pVDZ"])
+def test_uhf_one_electron_energy_is_lowest_eigenvalue(bset):
+    import scipy.linalg
+    atoms = [("H", (0.0, 0.0, 0.0))]
+    E, Pa, Pb, ok, (d, Sm, H, eri) = _uhf(atoms, bset, 1, 0)
+    assert ok
+    e0 = scipy.linalg.eigh(H, Sm, eigvals_only=True)[0]
+    assert abs(E - e0) < 1e-10
+    assert np.abs(Pb).max() == 0.0
+
+
+def _rotated(P, Sm, kappa_seed, h):
+    """Density after a rotation exp(h kappa) that mixes occupied and virtual
+    orbitals (orthonormal basis X = S^-1/2), kappa antisymmetric."""
+    import scipy.linalg
+    w, U = np.linalg.eigh(Sm)
+    Xh = U @ np.diag(np.sqrt(w)) @ U.T      # S^1/2
+    Pp = Xh @ P @ Xh                         # projector onto the occupied space
+    ev, V = np.linalg.eigh(Pp)
+    occ = ev > 0.5
+    Vo, Vv = V[:, occ], V[:, ~occ]
+    rng = np.random.default_rng(kappa_seed)
+    k = rng.standard_normal((Vo.shape[1], Vv.shape[1]))
+    A = Vv @ k.T @ Vo.T
+    A = A - A.T
+    R = scipy.linalg.expm(h * A)
+    Pr = R @ Pp @ R.T
+    Xm = U @ np.diag(1.0 / np.sqrt(w)) @ U.T
+    return Xm @ Pr @ Xm
+
+
+def _stationarity(E, Pa, Pb, ints, atoms):
+    d, Sm, H, eri = ints
+    K = Sm.shape[0]
+    g = orc.expand(eri, K)
+    E_nuc = nuclear_repulsion(atoms)
+    E_f = _functional(Pa, Pb, H, g, E_nuc)
+    h = 1e-4
+    grads = []
+    for seed in range(3):
+        Ep = _functional(_rotated(Pa, Sm, seed, h), _rotated(Pb, Sm, seed + 10, h), H, g, E_nuc)
+        Em = _functional(_rotated(Pa, Sm, seed, -h), _rotated(Pb, Sm, seed + 10, -h), H, g, E_nuc)
+        grads.append(abs(Ep - Em) / (2 * h))
+    return E_f, max(grads)
+
+
+@pytest.mark.parametrize("case", ["O2_triplet_STO-3G", "Li_6-31G*", "CH2_triplet_STO-3G"])
+def test_uhf_open_shell_is_stationary_point_of_energy_functional(case):
+    if case == "O2_triplet_STO-3G":
+        atoms, bset, na, nb = [("O", (-1.14, 0.0, 0.0)), ("O", (1.14, 0.0, 0.0))], "STO-3G", 9, 7
+    elif case == "Li_6-31G*":
+        atoms, bset, na, nb = [("Li", (0.0, 0.0, 0.0))], "6-31G*", 2, 1
+    else:
+        atoms, bset, na, nb = [("C", (0.0, 0.0, 0.0)), ("H", (1.6, 1.1, 0.0)), ("H", (-1.6, 1.1, 0.0))], "STO-3G", 5, 3
+    E, Pa, Pb, ok, ints = _uhf(atoms, bset, na, nb)
+    assert ok
+    E_f, grad = _stationarity(E, Pa, Pb, ints, atoms)
+    print(f"{case}: E={E:.12f} functional={E_f:.12f} max|dE/dtheta|={grad:.2e}")
+    assert abs(E - E_f) < 1e-10
+    assert grad < 1e-6
+    # idempotency and occupation of the converged densities
+    Sm = ints[1]
+    for P, n in ((Pa, na), (Pb, nb)):
+        assert abs(np.trace(P @ Sm) - n) < 1e-10
+        assert np.abs(P @ Sm @ P - P).max() < 1e-9
+
+
+def test_uhf_stationarity_check_detects_a_planted_mistake():
+    """The check has teeth: an oracle that swaps Ka and Kb in the Fock
+    matrices converges to densities that are not stationary for the true
+    functional."""
+    atoms, bset = [("O", (-1.14, 0.0, 0.0)), ("O", (1.14, 0.0, 0.0))], "STO-3G"
+    E, Pa, Pb, ok, ints = _uhf(atoms, bset, 9, 7, swap_k=True)
+    if not ok:
+        return   # not converging is a detection too
+    E_f, grad = _stationarity(E, Pa, Pb, ints, atoms)
+    print(f"planted: E={E:.12f} functional={E_f:.12f} grad={grad:.2e}")
+    assert grad > 1e-3 or abs(E - E_f) > 1e-6
