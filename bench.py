@@ -178,7 +178,7 @@ def direct_scf_pair(name, atoms, basis_set, thr=1e-12, split=1e-5):
 def stored_jk(hfx, N, basis, thr, steps):
     """J/K from the stored unique array (the per-iteration work of a stored SCF)
     on device buffers, timed with HIP events on the launch stream; HBM roofline
-    of the two passes over the array (k_jk_unique and k_j_columns)."""
+    of the one pass over the array (k_jk_rows) plus the K x K matrices."""
     import torch
     K, Q = basis.K, basis.n_quartets
     fb = hfx.FockBuilder(basis, mode="stored", threshold=thr)
@@ -205,9 +205,11 @@ def stored_jk(hfx, N, basis, thr, steps):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     fb.close()
-    hbm = 2 * 8 * Q   # the unique array read by both kernels; K^2 matrices are negligible
+    # algorithmic bytes of one RHF build: one pass over the Q-long FP64 unique
+    # array plus the K x K matrices (one density in; J and K out)
+    hbm = 8 * Q + 8 * K * K * (1 + 2)
     gbs = hbm / (ms * 1e-3) / 1e9
-    log(f"  stored J/K: {ms:.3f} ms per build, {gbs:.0f} GB/s")
+    log(f"  stored J/K: {ms:.3f} ms per build, {gbs:.0f} GB/s of algorithmic bytes")
     traffic = None   # measured HBM bytes per build (FETCH_SIZE / WRITE_SIZE runs, profiles/)
     tpath = os.path.join(ROOT, "profiles", "traffic_benzene_stored_jk.json")
     if os.path.exists(tpath):

@@ -146,9 +146,13 @@ struct hfx_basis {
 namespace {
 
 // Optional per-class timing of ERI launches (HIP events on the launch stream).
+// A measurement hook for one job at a time: while it is on, the events and
+// `used` flags belong to whichever build ran last; with it off (the default)
+// launch_plan touches nothing but the atomic flag, so concurrent builds from
+// several threads share no state here.
 struct ClassTimer {
-    bool on = false;
-    bool serial = false;   // every list on the caller's stream: each class has the device to itself
+    std::atomic<bool> on{false};
+    std::atomic<bool> serial{false};   // every list on the caller's stream: each class has the device to itself
     hipEvent_t ev[NQCLASS][2][2] = {};
     bool used[NQCLASS][2] = {};
     bool created = false;
@@ -253,7 +257,8 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
         const int v = e ? std::atoi(e) : NSIDE + 1;
         return v < 1 ? 1 : v > NSIDE + 1 ? NSIDE + 1 : v;
     }();
-    const bool fan = b->ev_fork && nlaunch > 1 && !(g_timer.on && g_timer.serial);
+    const bool timing = g_timer.on.load();
+    const bool fan = b->ev_fork && nlaunch > 1 && !(timing && g_timer.serial.load());
     hipStream_t lane_st[NSIDE + 1] = {st};
     for (int i = 0; i < NSIDE; ++i) lane_st[i + 1] = b->side[i];
     int order[NQCLASS * 2], nl = 0;
@@ -266,7 +271,8 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
     double load[NSIDE + 1] = {};
     bool used_side[NSIDE] = {};
     if (fan) HIPCHK(hipEventRecord(b->ev_fork, st));
-    for (int qc = 0; qc < NQCLASS; ++qc) g_timer.used[qc][0] = g_timer.used[qc][1] = false;
+    if (timing)
+        for (int qc = 0; qc < NQCLASS; ++qc) g_timer.used[qc][0] = g_timer.used[qc][1] = false;
     for (int k = 0; k < nl; ++k) {
         const int qc = order[k] / 2, p = order[k] % 2;
         int lane = 0;
@@ -281,7 +287,7 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
         }
         hipStream_t ls = lane_st[lane];
         {
-            if (g_timer.on) HIPCHK(hipEventRecord(g_timer.ev[qc][p][0], ls));
+            if (timing) HIPCHK(hipEventRecord(g_timer.ev[qc][p][0], ls));
             size_t off = dl.off[qc][p];
             for (int g = 0; g < NGROUP; ++g) {
                 const int n = dl.bucket[qc][p][g];
@@ -298,7 +304,7 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
                 HIPCHK(launch_eri_class(qc, p == 1, L, ls));
                 off += n;
             }
-            if (g_timer.on) {
+            if (timing) {
                 HIPCHK(hipEventRecord(g_timer.ev[qc][p][1], ls));
                 g_timer.used[qc][p] = true;
             }
@@ -424,8 +430,34 @@ int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets ta
     return 0;
 }
 
+// Waits for the streams a plan's launches may still run on (the caller's and
+// the basis' side streams) before their blocks go back to the cache after a
+// failure part-way through a launch sequence.
+void drain(const hfx_basis* b, hipStream_t st) {
+    if (st) (void)hipStreamSynchronize(st);
+    if (b->stream && b->stream != st) (void)hipStreamSynchronize(b->stream);
+    for (int i = 0; i < NSIDE; ++i)
+        if (b->side[i]) (void)hipStreamSynchronize(b->side[i]);
+}
+
+int compute_bounds_impl(hfx_basis* b);
+
+// Host work reached from the C ABI turns allocation failures and other
+// exceptions into status codes instead of terminating the process.
 int compute_bounds(hfx_basis* b) {
     if (b->bounds_ready) return 0;
+    try {
+        return compute_bounds_impl(b);
+    } catch (const std::bad_alloc&) {
+        drain(b, b->stream);
+        return fail(HFX_ERR_NOMEM, "host memory for the Schwarz bounds");
+    } catch (const std::exception& e) {
+        drain(b, b->stream);
+        return fail(HFX_ERR_DEVICE, std::string("Schwarz bounds: ") + e.what());
+    }
+}
+
+int compute_bounds_impl(hfx_basis* b) {
     const int P = b->plan.K * (b->plan.K + 1) / 2;
     QuartetLists ql;
     std::vector<uint2> diag[NQCLASS];
@@ -453,6 +485,7 @@ int compute_bounds(hfx_basis* b) {
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("schwarz bounds: ") + hipGetErrorString(e));
     }
+    if (rc) drain(b, b->stream);
     dfree(dl.d);
     if (rc) return rc;
     shell_pair_bounds(b->plan, b->bounds.data(), b->Qsh);
@@ -492,7 +525,7 @@ int jk_from_unique(hfx_basis* b, const double* d_eri, const double* d_Dj, const 
     const int K = b->plan.K;
     const size_t K2 = (size_t)K * K;
     HIPCHK(zero_jk(d_J, d_Ka, d_Kb, nspin, K2, st));
-    HIPCHK(launch_jk_unique(d_eri, b->d_fnpairs, K, nspin, d_Dj, d_Da, d_Db, d_J, d_Ka, d_Kb, st));
+    HIPCHK(launch_jk_rows(d_eri, b->d_fnpairs, K, nspin, d_Dj, d_Da, d_Db, d_J, d_Ka, d_Kb, st));
     HIPCHK(launch_symmetrize_n(d_J, d_Ka, d_Kb, 1 + nspin, K, st));
     return 0;
 }
@@ -673,6 +706,7 @@ int hfx_eri_unique(hfx_basis* b, double threshold, double* out, size_t Q) {
     if (int rc = dalloc(&d, Q)) return rc;
     int rc = eri_unique_into(b, threshold, d, b->stream);
     if (rc == 0) rc = copy_to_host(out, d, Q, b->stream);
+    if (rc) drain(b, b->stream);
     dfree(d);
     return rc;
 }
@@ -702,6 +736,7 @@ int hfx_coulomb_exchange(hfx_basis* b, const double* d_eri, const double* Pa, co
         if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("J/K: ") + hipGetErrorString(e));
     }
+    if (rc) drain(b, b->stream);
     dfree(buf);
     return rc;
 }
@@ -791,6 +826,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
         dfree(f->plan.d);
     }
     if (rc) {
+        drain(b, b->stream);
         hfx_fock_destroy(f);
         return rc;
     }
@@ -1135,8 +1171,8 @@ int hfx_set_class_timing(int on) {
                 for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreate(&g_timer.ev[qc][p][k]));
         g_timer.created = true;
     }
-    g_timer.on = on != 0;
-    g_timer.serial = on == 2;
+    g_timer.serial.store(on == 2);
+    g_timer.on.store(on != 0);
     return 0;
 }
 
@@ -1154,15 +1190,22 @@ int hfx_plan_stats(hfx_basis* b, double threshold, int precision, double mixed_s
                    long long* nprimq) {
     if (!b) return fail(HFX_ERR_INVALID, "null basis");
     if (int rc = compute_bounds(b)) return rc;
-    QuartetLists ql;
-    plan_quartets(b->plan, b->Qsh, threshold, precision == HFX_PREC_MIXED, mixed_split > 0 ? mixed_split : 1e-5, ql);
-    for (int qc = 0; qc < NQCLASS; ++qc)
-        for (int p = 0; p < 2; ++p) {
-            long long s = 0;
-            for (const uint2& e : ql.list[qc][p]) s += (long long)b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim;
-            if (nq) nq[2 * qc + p] = (long long)ql.list[qc][p].size();
-            if (nprimq) nprimq[2 * qc + p] = s;
-        }
+    try {
+        QuartetLists ql;
+        plan_quartets(b->plan, b->Qsh, threshold, precision == HFX_PREC_MIXED, mixed_split > 0 ? mixed_split : 1e-5,
+                      ql);
+        for (int qc = 0; qc < NQCLASS; ++qc)
+            for (int p = 0; p < 2; ++p) {
+                long long s = 0;
+                for (const uint2& e : ql.list[qc][p]) s += (long long)b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim;
+                if (nq) nq[2 * qc + p] = (long long)ql.list[qc][p].size();
+                if (nprimq) nprimq[2 * qc + p] = s;
+            }
+    } catch (const std::bad_alloc&) {
+        return fail(HFX_ERR_NOMEM, "host memory for the quartet plan");
+    } catch (const std::exception& e) {
+        return fail(HFX_ERR_DEVICE, std::string("quartet plan: ") + e.what());
+    }
     return 0;
 }
 

@@ -21,8 +21,9 @@ using LaunchFn = hipError_t (*)(const ClassLaunch&, hipStream_t);
 
 hipError_t launch_eri_class(int qc, bool fp32, const ClassLaunch& L, hipStream_t st);
 bool class_uses_family_a(int qc);
-hipError_t launch_jk_unique(const double* eri, const int2* pairs, int K, int nspin, const double* Dj, const double* Da,
-                            const double* Db, double* J, double* Ka, double* Kb, hipStream_t st);
+// one-pass J/K from the stored unique array (jk_stored.hip); Jacc / Kacc must be zeroed
+hipError_t launch_jk_rows(const double* eri, const int2* pairs, int K, int nspin, const double* Dj, const double* Da,
+                          const double* Db, double* J, double* Ka, double* Kb, hipStream_t st);
 hipError_t launch_symmetrize(double* M, int K, hipStream_t st);
 hipError_t launch_symmetrize_n(double* M0, double* M1, double* M2, int nmat, int K, hipStream_t st);   // nmat <= 3
 hipError_t launch_zero(double* p, size_t n, hipStream_t st);

@@ -10,8 +10,8 @@ Schwarz threshold 1e-12, plus the sampled molecules of test_gpu_eri
     Boys function is not noise, and the exact-Boys oracle to 1e-12 everywhere;
   * the whole Q-long array, on the device: exactly 0.0 wherever the
     function-level Schwarz product b_IJ b_KL is below the threshold
-    (integrals.hpp:771-776) and, for the jittered molecules (no symmetry
-    zeros), non-zero everywhere else, so a write to a wrong eri_index slot
+    (integrals.hpp:771-776), and elsewhere zero only where the oracle says
+    the integral vanishes by symmetry, so a write to a wrong eri_index slot
     (indexing.hpp:42-52) cannot hide outside the sample;
   * J / K of the direct build (ERIs contracted in the kernel epilogue, no
     unique array) equal J / K streamed from the stored unique array to 1e-11
@@ -51,13 +51,14 @@ def _device_unique(gpu, b, thr):
     return out
 
 
-def _mask_checks(eri, bounds, thr, nonzero_expected, label):
-    """Walks the device array in chunks: zero where masked; when asked, no
-    zero where unmasked.  Returns (n_masked, n_zero_unmasked)."""
+def _mask_checks(eri, bounds, thr, label):
+    """Walks the device array in chunks.  Returns (n_masked, n_written_where
+    masked, ids of entries that are exactly zero where the mask keeps them)."""
     import torch
     bd = torch.from_numpy(bounds).cuda()
     Q = eri.numel()
-    n_masked = n_zero_kept = n_bad = 0
+    n_masked = n_bad = 0
+    zero_kept = []
     step = 1 << 25
     for lo in range(0, Q, step):
         q = torch.arange(lo, min(Q, lo + step), dtype=torch.int64, device="cuda")
@@ -69,12 +70,11 @@ def _mask_checks(eri, bounds, thr, nonzero_expected, label):
         v = eri[lo:lo + q.numel()]
         n_bad += int(((~keep) & (v != 0)).sum())
         n_masked += int((~keep).sum())
-        n_zero_kept += int((keep & (v == 0)).sum())
-    print(f"{label}: Q={Q} masked={n_masked} written-where-masked={n_bad} zero-where-kept={n_zero_kept}", flush=True)
-    assert n_bad == 0
-    if nonzero_expected:
-        assert n_zero_kept == 0
-    return n_masked, n_zero_kept
+        zero_kept.append(q[keep & (v == 0)].cpu().numpy())
+    zero_kept = np.concatenate(zero_kept)
+    print(f"{label}: Q={Q} masked={n_masked} written-where-masked={n_bad} zero-where-kept={len(zero_kept)}",
+          flush=True)
+    return n_masked, n_bad, zero_kept
 
 
 @pytest.mark.parametrize("name", CONFIGS)
@@ -109,8 +109,19 @@ def test_whole_array_zero_pattern(gpu, name):
     bounds = gpu.schwarz_bounds(b)
     # the library's bounds against the reference's (Boys noise only)
     assert np.abs(bounds - d["bounds"]).max() < 1e-10
-    jittered = name in CONFIGS   # no point-group symmetry: no integral vanishes exactly
-    _mask_checks(eri, bounds, thr, jittered, name)
+    n_masked, n_bad, zero_kept = _mask_checks(eri, bounds, thr, name)
+    assert n_bad == 0
+    # a kept entry may be exactly zero only where the integral vanishes by
+    # symmetry (point group, or the local symmetry of one- and two-centre
+    # integrals, which the jittered molecules keep): the oracle's exact-Boys
+    # value there is zero too (seeded sample of up to 400 such entries)
+    if name in CONFIGS:   # jittered: no point-group zeros, only the few local ones
+        assert len(zero_kept) < 0.01 * (eri.numel() - n_masked)
+    if len(zero_kept):
+        pick = np.random.default_rng(5).choice(len(zero_kept), min(400, len(zero_kept)), replace=False)
+        ex = orc.eri_quartets(d, thr, zero_kept[pick], bounds=bounds, exact=True)
+        print(f"{name}: max |oracle| at zero-kept entries {np.abs(ex).max():.1e}", flush=True)
+        assert np.abs(ex).max() < 1e-15
 
 
 def _dens(K, seed):
