@@ -741,6 +741,68 @@ int hfx_coulomb_exchange(hfx_basis* b, const double* d_eri, const double* Pa, co
     return rc;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free J/K
+// Scratch of hfx_coulomb_exchange_noalloc: the J density Pa + Pb of a UHF
+// build (K x K doubles, K <= HFX_NOALLOC_MAX_K).  Each device holds its own
+// copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device.
+namespace {
+
+constexpr int NOALLOC_MAX_K = HFX_NOALLOC_MAX_K;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[(size_t)NOALLOC_MAX_K * NOALLOC_MAX_K];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current.
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(HFX_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(HFX_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfx_coulomb_exchange_noalloc(hfx_basis* b, const double* d_eri, const double* d_Pa, const double* d_Pb,
+                                 double* d_J, double* d_Ka, double* d_Kb, void* stream) {
+    if (!b || !d_eri || !d_Pa || !d_J || !d_Ka) return fail(HFX_ERR_INVALID, "null argument");
+    if (d_Pb && !d_Kb) return fail(HFX_ERR_INVALID, "UHF build needs Kb");
+    const int K = b->plan.K;
+    const int nspin = d_Pb ? 2 : 1;
+    if (nspin == 2 && K > NOALLOC_MAX_K)
+        return fail(HFX_ERR_UNSUPPORTED, "hfx_coulomb_exchange_noalloc: K = " + std::to_string(K) +
+                                             " exceeds the scratch maximum of " + std::to_string(NOALLOC_MAX_K) +
+                                             " functions for UHF");
+    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    double* dj = nullptr;
+    if (int rc = noalloc_scratch(&dj)) return rc;
+    const size_t K2 = (size_t)K * K;
+    const double* Dj = d_Pa;
+    if (nspin == 2) {   // every call rewrites the K x K part of the scratch it reads
+        HIPCHK(launch_add(dj, d_Pa, d_Pb, K2, st));
+        Dj = dj;
+    }
+    HIPCHK(zero_jk(d_J, d_Ka, d_Kb, nspin, K2, st));
+    HIPCHK(launch_jk_rows(d_eri, b->d_fnpairs, K, nspin, Dj, d_Pa, d_Pb, d_J, d_Ka, d_Kb, st));
+    HIPCHK(launch_symmetrize_n(d_J, d_Ka, d_Kb, 1 + nspin, K, st));
+    return 0;
+}
+
 int hfx_repulsion_matrix(hfx_basis* b, const double* d_eri, const double* P, double* G) {
     if (!b || !G) return fail(HFX_ERR_INVALID, "null argument");
     const int K = b->plan.K;

@@ -103,6 +103,22 @@ int hfx_one_electron_device(int K, const double* xyz, const int* lmn, const int*
 int hfx_coulomb_exchange(hfx_basis* b, const double* d_eri, const double* Pa, const double* Pb, double* J,
                          double* Ka, double* Kb);
 
+/* No counterpart in the reference: an allocation-free variant of
+ * hfx_coulomb_exchange for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Densities in, J / Ka / Kb out, all caller-owned device K x K
+ * matrices; enqueued on `stream` (a hipStream_t, NULL = the basis' stream),
+ * asynchronous.  RHF: d_Pb == NULL (J and Ka from Pa).  UHF: J from Pa + Pb,
+ * which is formed in a fixed device scratch array of HFX_NOALLOC_MAX_K^2
+ * doubles (2 MiB per GPU, held until the process ends); a UHF call with
+ * K > HFX_NOALLOC_MAX_K returns HFX_ERR_UNSUPPORTED.  The array is shared by
+ * every call on one GPU: UHF calls that overlap on one GPU (different streams
+ * or threads) race on it.  No hipMalloc / hipFree or allocating library call
+ * is made. */
+#define HFX_NOALLOC_MAX_K 512
+int hfx_coulomb_exchange_noalloc(hfx_basis* b, const double* d_eri, const double* d_Pa, const double* d_Pb,
+                                 double* d_J, double* d_Ka, double* d_Kb, void* stream);
+
 typedef struct {
     int mode;              /* HFX_MODE_* */
     int precision;         /* HFX_PREC_* */
