@@ -255,7 +255,7 @@ def benzene_energy_fixture(L):
     print("reference unique array", time.time() - t0, "s", flush=True)
     E, Pm, Cm, e, it, ok = scf_oracle.rhf(S, T + V, E_nuc, ne, lambda Pd: orc.repulsion_matrix(Pd, eri_ref), eps=1e-10)
     assert ok
-    res["E_reference_eris"] = repr(E)
+    res["E_reference_eris"] = float(E)
     res["iterations_reference_eris"] = it
     del eri_ref
     bd_x = orc.schwarz_bounds_exact(d)
@@ -264,7 +264,7 @@ def benzene_energy_fixture(L):
     print("exact-Boys unique array", time.time() - t0, "s", flush=True)
     E2, Pm2, Cm2, e2, it2, ok2 = scf_oracle.rhf(S, T + V, E_nuc, ne, lambda Pd: orc.repulsion_matrix(Pd, eri_x), eps=1e-10)
     assert ok2
-    res["E_exact_eris"] = repr(E2)
+    res["E_exact_eris"] = float(E2)
     res["iterations_exact_eris"] = it2
     res["note"] = ("E_reference_eris: numpy SCF oracle on the reference's own unique ERIs and S/T/V "
                    "(include/integrals.hpp compiled in place); E_exact_eris: the same on the oracle's "

@@ -39,12 +39,16 @@ def _oracle_rhf(d):
 
 @pytest.mark.parametrize("name", ["h2o_sto3g.npz", "h2o_ccpvdz.npz"])
 @pytest.mark.parametrize("mode", ["stored", "direct"])
-def test_rhf_matches_oracle_on_golden_integrals(gpu, name, mode):
+@pytest.mark.parametrize("device_iteration", ["off", "on"])
+def test_rhf_matches_oracle_on_golden_integrals(gpu, name, mode, device_iteration):
+    """Host iteration and the GPU-resident iteration (Fock assembly, DIIS,
+    transforms and densities on the device) against the oracle directly."""
     from hartree_fock_cpp_amd.scf import rhf_diis
     d = golden(name)
     E_ref, P_ref = _oracle_rhf(d)
-    r = rhf_diis(_config(d), str(d["basis_set"]), epsilon=1e-10, eri_mode=mode)
-    print(f"{name} {mode}: E={r['E_tot']:.12f} oracle={E_ref:.12f} it={r['iterations']}")
+    r = rhf_diis(_config(d), str(d["basis_set"]), epsilon=1e-10, eri_mode=mode, device_iteration=device_iteration)
+    print(f"{name} {mode} device_iteration={device_iteration}: E={r['E_tot']:.12f} oracle={E_ref:.12f} "
+          f"it={r['iterations']}")
     assert r["converged"]
     assert abs(r["E_tot"] - E_ref) < TOL_E
     assert np.abs(r["density"] - P_ref).max() < 1e-6
@@ -66,9 +70,12 @@ def test_rhf_reproduces_reference_energy_curves(gpu, row):
 
 
 def test_uhf_o2_triplet_matches_oracle(gpu):
-    """UHF O2 / cc-pVTZ (R = 2.28 bohr, 9 alpha / 7 beta): native driver vs
-    the numpy UHF oracle fed with the GPU unique ERIs (themselves checked
-    against the reference in test_gpu_eri) and the golden S, T, V."""
+    """UHF O2 / cc-pVTZ (R = 2.28 bohr, 9 alpha / 7 beta): native driver, host
+    and GPU-resident iterations, vs the numpy UHF oracle (pinned in
+    test_oracle_scf by the RHF limit, the one-electron eigenvalue and the
+    stationarity of the UHF energy functional) fed with the GPU unique ERIs
+    (themselves checked against the reference in test_gpu_eri) and the golden
+    S, T, V."""
     from hartree_fock_cpp_amd.scf import uhf_diis
     from hartree_fock_cpp_amd.workloads import nuclear_repulsion
     d = golden("o2_ccpvtz.npz")
@@ -83,10 +90,12 @@ def test_uhf_o2_triplet_matches_oracle(gpu):
                                                    eps=1e-9)
     assert ok
     for mode in ("stored", "direct"):
-        r = uhf_diis(cfg, "cc-pVTZ", (9, 7), epsilon=1e-9, eri_mode=mode)
-        print(f"O2 UHF {mode}: E={r['E_tot']:.12f} oracle={E_ref:.12f} it={r['iterations']}")
-        assert r["converged"]
-        assert abs(r["E_tot"] - E_ref) < TOL_E
+        for dev in ("off", "on"):
+            r = uhf_diis(cfg, "cc-pVTZ", (9, 7), epsilon=1e-9, eri_mode=mode, device_iteration=dev)
+            print(f"O2 UHF {mode} device_iteration={dev}: E={r['E_tot']:.12f} oracle={E_ref:.12f} "
+                  f"it={r['iterations']}")
+            assert r["converged"]
+            assert abs(r["E_tot"] - E_ref) < TOL_E
 
 
 def test_mixed_precision_with_polish(gpu):
@@ -137,14 +146,24 @@ def test_incremental_fock_scf_matches_full(gpu):
         assert np.abs(r0["density"] - r1["density"]).max() < 1e-7
 
 
-def test_benzene_ccpvdz_full_size_properties(gpu):
-    """Benzene RHF/cc-pVDZ at full size: stored and direct agree, the density
-    is idempotent (P S P = 2 P) and holds 42 electrons."""
-    from hartree_fock_cpp_amd import workloads as W
+def test_benzene_ccpvdz_energy_and_properties(gpu):
+    """Benzene RHF/cc-pVDZ at full size (the configuration of the headline):
+    the total energy equals the numpy SCF oracle's on the reference's own
+    unique ERIs and S/T/V (tests/golden/benzene_ccpvdz_energy.json, computed
+    from include/integrals.hpp compiled in place) and on the oracle's
+    exact-Boys ERIs, to 1e-8 Ha; stored and direct agree, the density is
+    idempotent (P S P = 2 P) and holds 42 electrons."""
+    import json
     from hartree_fock_cpp_amd.scf import one_electron, rhf_diis
-    cfg = W.benzene()
-    rs = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-8, eri_mode="stored")
-    rd = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-8, eri_mode="direct")
+    with open(os.path.join(GOLDEN, "benzene_ccpvdz_energy.json")) as f:
+        ref = json.load(f)
+    cfg = [(s, tuple(x)) for s, x in ref["geometry"]]
+    rs = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-10, eri_mode="stored", screening_threshold=ref["threshold"])
+    rd = rhf_diis(cfg, "cc-pVDZ", epsilon=1e-10, eri_mode="direct", screening_threshold=ref["threshold"])
+    print(f"benzene: E={rs['E_tot']:.11f} reference-ERI oracle {ref['E_reference_eris']:.11f} "
+          f"exact-ERI oracle {ref['E_exact_eris']:.11f}")
+    assert abs(rs["E_tot"] - ref["E_exact_eris"]) < TOL_E
+    assert abs(rs["E_tot"] - ref["E_reference_eris"]) < TOL_E
     print(f"benzene: stored E={rs['E_tot']:.10f} ({rs['t_total']:.2f} s, {rs['iterations']} it) "
           f"direct E={rd['E_tot']:.10f} ({rd['t_total']:.2f} s)")
     assert rs["converged"] and rd["converged"]
@@ -153,4 +172,3 @@ def test_benzene_ccpvdz_full_size_properties(gpu):
     P = rs["density"]
     assert abs(np.trace(P @ S) - 42.0) < 1e-8
     assert np.abs(P @ S @ P - 2.0 * P).max() < 1e-7
-    assert -231.0 < rs["E_tot"] < -230.0
