@@ -417,7 +417,7 @@ def main():
                    "shell_quartets": int(nq.sum()), "primitive_quartets": int(nprimq.sum()),
                    "model_gflop_per_build": flops_cls.sum() / 1e9},
         "roofline": {"bound": "mfma", "pipe": "fp64 (VALU; same 78.6 TFLOP/s peak as FP64 MFMA)",
-                     "kernel": "%s %s fp%s" % (W.kernel_family(dom // 2), W.qclass_name(dom // 2),
+                     "kernel": "%s %s fp%s" % (W.kernel_family(dom // 2, dom % 2 == 1), W.qclass_name(dom // 2),
                                                "32" if dom % 2 else "64"),
                      "achieved": dom_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": dom_tflops / PEAK_FP64_TFLOPS, "traffic": traffic,

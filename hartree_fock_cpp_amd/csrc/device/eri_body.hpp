@@ -58,6 +58,24 @@ __host__ __device__ constexpr bool use_family_a(int la, int lb, int lc, int ld) 
     return L <= 4 && nout + nherm(L) <= 128;
 }
 
+// Family M (eri_mfma.hpp: both Hermite -> Cartesian contractions on the
+// matrix cores) runs the classes whose ket Hermite and Cartesian sizes (at
+// least 4 and 6) keep the MFMA tiles from being mostly padding: every such
+// class beyond family A's reach, and the family-A class (pp|pp), which it runs
+// 2x faster in the benzene / cc-pVDZ unique build (measured; (ds|ds) ran 1.5x
+// slower there and stays in family A).
+__host__ __device__ constexpr bool use_family_m(int la, int lb, int lc, int ld) {
+    if (nherm(lc + ld) < 4 || ncart(lc) * ncart(ld) < 6) return false;
+    if (!use_family_a(la, lb, lc, ld)) return true;
+    return la == 1 && lb == 1 && lc == 1 && ld == 1;
+}
+
+// The class runs the family-A kernel (else family B or M, which share the
+// work-unit semantics of famb_range).
+__host__ __device__ constexpr bool runs_family_a(int la, int lb, int lc, int ld) {
+    return use_family_a(la, lb, lc, ld) && !use_family_m(la, lb, lc, ld);
+}
+
 // ------------------------------------------------------------------ epilogue
 // A shell quartet block v[a][b][c][d] (already scaled to final values) for the
 // kernel-order shells (s1 s2 | s3 s4) is consumed in one of three ways.  The
@@ -520,9 +538,8 @@ struct FamBSmem {
 // Stages what the epilogue of one quartet reads: function scales, pair bounds
 // (not for EPI_DIAG, which is producing them) and, for J/K, the density
 // blocks, so the epilogue reads LDS instead of HBM / L2.
-template <int LA, int LB, int LC, int LD, class Real, bool JK>
-HFX_HD __forceinline__ void famb_stage_epi(FamBSmem<LA, LB, LC, LD, Real, JK>& sm, const QuartetCtx& q,
-                                           const EpiArgs& ep, int lane, int nlanes) {
+template <int LA, int LB, int LC, int LD, class Real, bool JK, class SM>
+HFX_HD __forceinline__ void famb_stage_epi(SM& sm, const QuartetCtx& q, const EpiArgs& ep, int lane, int nlanes) {
     using S = FamBShape<LA, LB, LC, LD>;
     double* E = sm.epi;
     const int K = ep.K;
@@ -632,6 +649,115 @@ HFX_HD __forceinline__ void build_e_lanes(Real* dst, const PrimPair<Real>& pp, b
                 for (int t = 0; t <= LA + LB; ++t)
                     dst[x * S + (a * (LB + 1) + b) * (LA + LB + 1) + t] = (sign && (t & 1)) ? -h.e[a][b][t] : h.e[a][b][t];
     });
+}
+
+// Epilogue of one chunk of ket component pairs cd0 .. cd0 + ncd - 1 of a
+// family-B / family-M work unit: the raw block values sit in
+// sm.acc[ab * CDC + cd - cd0]; sm.epi holds the staged scales, bounds and
+// densities (famb_stage_epi).  Shared by the VALU and the MFMA kernels.
+template <int LA, int LB, int LC, int LD, int CDC, bool JK, class Real, class SM, class Sync>
+HFX_HD void famb_epilogue(SM& sm, const QuartetCtx& q, const EpiArgs& ep, int cd0, int ncd, int nslice, double fdeg,
+                          int lane, int nlanes, Sync sync) {
+    using S = FamBShape<LA, LB, LC, LD>;
+    const int K = ep.K;
+    const double* E = sm.epi;
+    auto fs4 = [&](int ab, int cd) -> double {
+        return E[S::O_FS + ab / S::NB] * E[S::O_FS + S::NA + ab % S::NB] *
+               E[S::O_FS + S::NA + S::NB + cd / S::ND] * E[S::O_FS + S::NA + S::NB + S::NC + cd % S::ND];
+    };
+    if (!JK && ep.mode == EPI_UNIQUE) {
+        for (int w = lane; w < S::NAB * ncd; w += nlanes) {
+            const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
+            const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;
+            const int la = q.fn3 + cd / S::ND, si = q.fn4 + cd % S::ND;
+            if (!keep_unique(q, mu, nu, la, si)) continue;
+            if (E[S::O_BAB + ab] * E[S::O_BCD + cd] < ep.threshold) continue;
+            const double v = (double)sm.acc[ab * CDC + cdl] * fs4(ab, cd);
+            const int64_t o = quartet_of(pair_of(mu, nu), pair_of(la, si));
+            if (nslice > 1)
+                atomic_add(&ep.out[o], v);
+            else
+                ep.out[o] = v;
+        }
+    } else if (!JK && ep.mode == EPI_DIAG) {
+        for (int w = lane; w < S::NAB; w += nlanes) {
+            const int cd = w;   // (ab|ab): same component pair on both sides
+            if (cd < cd0 || cd >= cd0 + ncd) continue;
+            const int mu = q.fn1 + w / S::NB, nu = q.fn2 + w % S::NB;
+            if (q.d12 && mu < nu) continue;
+            const double v = (double)sm.acc[w * CDC + (cd - cd0)] * fs4(w, cd);
+            ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
+        }
+    } else if constexpr (JK) {
+        // final masked values in place, then every J/K sum reads LDS only
+        for (int w = lane; w < S::NAB * ncd; w += nlanes) {
+            const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
+            Real& a = sm.acc[ab * CDC + cdl];
+            a = E[S::O_BAB + ab] * E[S::O_BCD + cd] >= ep.threshold ? (Real)((double)a * fs4(ab, cd)) : Real(0);
+        }
+        sync();
+        auto v = [&](int ab, int cdl) -> double { return (double)sm.acc[ab * CDC + cdl]; };
+        // J_ab over this chunk's cd
+        for (int ab = lane; ab < S::NAB; ab += nlanes) {
+            double s = 0.0;
+            for (int cdl = 0; cdl < ncd; ++cdl) s += v(ab, cdl) * E[S::O_DJCD + cd0 + cdl];
+            atomic_add(&ep.J[(q.fn1 + ab / S::NB) * K + q.fn2 + ab % S::NB], 2.0 * fdeg * s);
+        }
+        for (int cdl = lane; cdl < ncd; cdl += nlanes) {
+            const int cd = cd0 + cdl;
+            double s = 0.0;
+            for (int ab = 0; ab < S::NAB; ++ab) s += v(ab, cdl) * E[S::O_DJAB + ab];
+            atomic_add(&ep.J[(q.fn3 + cd / S::ND) * K + q.fn4 + cd % S::ND], 2.0 * fdeg * s);
+        }
+        for (int spin = 0; spin < ep.nspin; ++spin) {
+            const double* dk = E + S::O_DK + spin * S::NDK;
+            const double* Dac = dk;
+            const double* Dad = Dac + S::NA * S::NC;
+            const double* Dbc = Dad + S::NA * S::ND;
+            const double* Dbd = Dbc + S::NB * S::NC;
+            double* Kacc = spin == 0 ? ep.Ka : ep.Kb;
+            // Kacc_ac += f sum_bd v D_bd ; Kacc_bc += f sum_ad v D_ad : targets (x, c)
+            for (int w = lane; w < (S::NA + S::NB) * S::NC; w += nlanes) {
+                const bool isA = w < S::NA * S::NC;
+                const int x = isA ? w / S::NC : (w - S::NA * S::NC) / S::NC;
+                const int c = w % S::NC;
+                double s = 0.0;
+                bool any = false;
+                for (int d = 0; d < S::ND; ++d) {
+                    const int cd = c * S::ND + d;
+                    if (cd < cd0 || cd >= cd0 + ncd) continue;
+                    any = true;
+                    const int cdl = cd - cd0;
+                    if (isA) {
+                        for (int bb = 0; bb < S::NB; ++bb) s += v(x * S::NB + bb, cdl) * Dbd[bb * S::ND + d];
+                    } else {
+                        for (int a = 0; a < S::NA; ++a) s += v(a * S::NB + x, cdl) * Dad[a * S::ND + d];
+                    }
+                }
+                if (any) atomic_add(&Kacc[((isA ? q.fn1 : q.fn2) + x) * K + q.fn3 + c], fdeg * s);
+            }
+            // Kacc_ad += f sum_bc v D_bc ; Kacc_bd += f sum_ac v D_ac : targets (x, d)
+            for (int w = lane; w < (S::NA + S::NB) * S::ND; w += nlanes) {
+                const bool isA = w < S::NA * S::ND;
+                const int x = isA ? w / S::ND : (w - S::NA * S::ND) / S::ND;
+                const int d = w % S::ND;
+                double s = 0.0;
+                bool any = false;
+                for (int c = 0; c < S::NC; ++c) {
+                    const int cd = c * S::ND + d;
+                    if (cd < cd0 || cd >= cd0 + ncd) continue;
+                    any = true;
+                    const int cdl = cd - cd0;
+                    if (isA) {
+                        for (int bb = 0; bb < S::NB; ++bb) s += v(x * S::NB + bb, cdl) * Dbc[bb * S::NC + c];
+                    } else {
+                        for (int a = 0; a < S::NA; ++a) s += v(a * S::NB + x, cdl) * Dac[a * S::NC + c];
+                    }
+                }
+                if (any) atomic_add(&Kacc[((isA ? q.fn1 : q.fn2) + x) * K + q.fn4 + d], fdeg * s);
+            }
+        }
+    }
 }
 
 // One wavefront (or one CPU thread with nlanes = 1) computes one work unit: a
@@ -828,104 +954,7 @@ HFX_HD void family_b_one(uint2 e, int chunk, const PairHdr* hdr, const PrimPair<
         }
         sync();
         // ---------------- epilogue for this chunk (values read from sm.acc)
-        const double* E = sm.epi;
-        auto fs4 = [&](int ab, int cd) -> double {
-            return E[S::O_FS + ab / S::NB] * E[S::O_FS + S::NA + ab % S::NB] *
-                   E[S::O_FS + S::NA + S::NB + cd / S::ND] * E[S::O_FS + S::NA + S::NB + S::NC + cd % S::ND];
-        };
-        if (!JK && ep.mode == EPI_UNIQUE) {
-            for (int w = lane; w < S::NAB * ncd; w += nlanes) {
-                const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
-                const int mu = q.fn1 + ab / S::NB, nu = q.fn2 + ab % S::NB;
-                const int la = q.fn3 + cd / S::ND, si = q.fn4 + cd % S::ND;
-                if (!keep_unique(q, mu, nu, la, si)) continue;
-                if (E[S::O_BAB + ab] * E[S::O_BCD + cd] < ep.threshold) continue;
-                const double v = (double)sm.acc[ab * S::CDC + cdl] * fs4(ab, cd);
-                const int64_t o = quartet_of(pair_of(mu, nu), pair_of(la, si));
-                if (nslice > 1)
-                    atomic_add(&ep.out[o], v);
-                else
-                    ep.out[o] = v;
-            }
-        } else if (!JK && ep.mode == EPI_DIAG) {
-            for (int w = lane; w < S::NAB; w += nlanes) {
-                const int cd = w;   // (ab|ab): same component pair on both sides
-                if (cd < cd0 || cd >= cd0 + ncd) continue;
-                const int mu = q.fn1 + w / S::NB, nu = q.fn2 + w % S::NB;
-                if (q.d12 && mu < nu) continue;
-                const double v = (double)sm.acc[w * S::CDC + (cd - cd0)] * fs4(w, cd);
-                ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
-            }
-        } else if constexpr (JK) {
-            // final masked values in place, then every J/K sum reads LDS only
-            for (int w = lane; w < S::NAB * ncd; w += nlanes) {
-                const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
-                Real& a = sm.acc[ab * S::CDC + cdl];
-                a = E[S::O_BAB + ab] * E[S::O_BCD + cd] >= ep.threshold ? (Real)((double)a * fs4(ab, cd)) : Real(0);
-            }
-            sync();
-            auto v = [&](int ab, int cdl) -> double { return (double)sm.acc[ab * S::CDC + cdl]; };
-            // J_ab over this chunk's cd
-            for (int ab = lane; ab < S::NAB; ab += nlanes) {
-                double s = 0.0;
-                for (int cdl = 0; cdl < ncd; ++cdl) s += v(ab, cdl) * E[S::O_DJCD + cd0 + cdl];
-                atomic_add(&ep.J[(q.fn1 + ab / S::NB) * K + q.fn2 + ab % S::NB], 2.0 * fdeg * s);
-            }
-            for (int cdl = lane; cdl < ncd; cdl += nlanes) {
-                const int cd = cd0 + cdl;
-                double s = 0.0;
-                for (int ab = 0; ab < S::NAB; ++ab) s += v(ab, cdl) * E[S::O_DJAB + ab];
-                atomic_add(&ep.J[(q.fn3 + cd / S::ND) * K + q.fn4 + cd % S::ND], 2.0 * fdeg * s);
-            }
-            for (int spin = 0; spin < ep.nspin; ++spin) {
-                const double* dk = E + S::O_DK + spin * S::NDK;
-                const double* Dac = dk;
-                const double* Dad = Dac + S::NA * S::NC;
-                const double* Dbc = Dad + S::NA * S::ND;
-                const double* Dbd = Dbc + S::NB * S::NC;
-                double* Kacc = spin == 0 ? ep.Ka : ep.Kb;
-                // Kacc_ac += f sum_bd v D_bd ; Kacc_bc += f sum_ad v D_ad : targets (x, c)
-                for (int w = lane; w < (S::NA + S::NB) * S::NC; w += nlanes) {
-                    const bool isA = w < S::NA * S::NC;
-                    const int x = isA ? w / S::NC : (w - S::NA * S::NC) / S::NC;
-                    const int c = w % S::NC;
-                    double s = 0.0;
-                    bool any = false;
-                    for (int d = 0; d < S::ND; ++d) {
-                        const int cd = c * S::ND + d;
-                        if (cd < cd0 || cd >= cd0 + ncd) continue;
-                        any = true;
-                        const int cdl = cd - cd0;
-                        if (isA) {
-                            for (int bb = 0; bb < S::NB; ++bb) s += v(x * S::NB + bb, cdl) * Dbd[bb * S::ND + d];
-                        } else {
-                            for (int a = 0; a < S::NA; ++a) s += v(a * S::NB + x, cdl) * Dad[a * S::ND + d];
-                        }
-                    }
-                    if (any) atomic_add(&Kacc[((isA ? q.fn1 : q.fn2) + x) * K + q.fn3 + c], fdeg * s);
-                }
-                // Kacc_ad += f sum_bc v D_bc ; Kacc_bd += f sum_ac v D_ac : targets (x, d)
-                for (int w = lane; w < (S::NA + S::NB) * S::ND; w += nlanes) {
-                    const bool isA = w < S::NA * S::ND;
-                    const int x = isA ? w / S::ND : (w - S::NA * S::ND) / S::ND;
-                    const int d = w % S::ND;
-                    double s = 0.0;
-                    bool any = false;
-                    for (int c = 0; c < S::NC; ++c) {
-                        const int cd = c * S::ND + d;
-                        if (cd < cd0 || cd >= cd0 + ncd) continue;
-                        any = true;
-                        const int cdl = cd - cd0;
-                        if (isA) {
-                            for (int bb = 0; bb < S::NB; ++bb) s += v(x * S::NB + bb, cdl) * Dbc[bb * S::NC + c];
-                        } else {
-                            for (int a = 0; a < S::NA; ++a) s += v(a * S::NB + x, cdl) * Dac[a * S::NC + c];
-                        }
-                    }
-                    if (any) atomic_add(&Kacc[((isA ? q.fn1 : q.fn2) + x) * K + q.fn4 + d], fdeg * s);
-                }
-            }
-        }
+        famb_epilogue<LA, LB, LC, LD, S::CDC, JK, Real>(sm, q, ep, cd0, ncd, nslice, fdeg, lane, nlanes, sync);
     }
     sync();
 }

@@ -5,10 +5,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <type_traits>
 #include <utility>
 
 #include "eri_body.hpp"
 #include "eri_classes.hpp"
+#include "eri_mfma.hpp"
 #include "hfx_launch.hpp"
 
 namespace hfx {
@@ -60,12 +63,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(famb_waves_p
                                                (int)threadIdx.x, 64, sync);
 }
 
+// One wavefront per work unit, grid-strided; class tables built once per workgroup.
+template <int LA, int LB, int LC, int LD, bool JK>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fam_m(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
+                                               const PrimPair<double>* __restrict__ pp,
+                                               const double* __restrict__ etab, const double* __restrict__ etabk,
+                                               const double* __restrict__ tab,
+                                               const HermTab* __restrict__ ht, EpiArgs ep) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __shared__ FamMSmem<LA, LB, LC, LD, double, JK> sm;
+    famm_tables<LA, LB, LC, LD>(sm, *ht, (int)threadIdx.x);
+    __syncthreads();
+    for (long long u = blockIdx.x; u < n; u += gridDim.x)
+        family_m_one<LA, LB, LC, LD, double, JK>(list[u], hdr, pp, etab, etabk, tab, ep, sm, (int)threadIdx.x);
+#endif
+}
+
+// Family M (MFMA contractions) for the FP64 kernels of the classes that
+// use_family_m admits; HFX_FAMM=0 in the environment selects family B
+// instead (measurement).
+inline bool famm_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("HFX_FAMM");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <int QC, class Real, bool JK>
 hipError_t launch_class_mode(const ClassLaunch& L, hipStream_t st) {
     using C = QClass<QC>;
     const PrimPair<Real>* pp = static_cast<const PrimPair<Real>*>(L.prim);
     const Real* tab = static_cast<const Real*>(L.boys);
-    if constexpr (use_family_a(C::LA, C::LB, C::LC, C::LD)) {
+    if constexpr (runs_family_a(C::LA, C::LB, C::LC, C::LD)) {
         const long long lanes = (long long)L.n * L.G;
         const int grid = (int)((lanes + FAM_A_BLOCK - 1) / FAM_A_BLOCK);
         switch (L.G) {
@@ -76,6 +106,15 @@ hipError_t launch_class_mode(const ClassLaunch& L, hipStream_t st) {
             default: return hipErrorInvalidValue;
         }
     } else {
+        if constexpr (std::is_same_v<Real, double> && use_family_m(C::LA, C::LB, C::LC, C::LD)) {
+            if (famm_enabled()) {
+                const int grid = L.n < 65536 ? L.n : 65536;
+                k_fam_m<C::LA, C::LB, C::LC, C::LD, JK><<<grid, 64, 0, st>>>(
+                    L.list, L.n, L.hdr, pp, static_cast<const double*>(L.etab), static_cast<const double*>(L.etabk), tab,
+                    L.herm, L.ep);
+                return hipGetLastError();
+            }
+        }
         const long long units = (long long)L.n * FamBShape<C::LA, C::LB, C::LC, C::LD>::NCHUNK;
         const int grid = units < 65536 ? (int)units : 65536;
         k_fam_b<C::LA, C::LB, C::LC, C::LD, Real, JK><<<grid, 64, 0, st>>>(L.list, L.n, L.hdr, pp, tab, L.herm, L.ep);

@@ -33,6 +33,7 @@ struct PairHdr {
     int nprim;         // number of kept primitive pairs
     int diag;          // 1 when s1 == s2
     int canon;         // canonical pair index A(A+1)/2 + B
+    long long eoff;    // first entry of the pair's Hermite -> Cartesian matrices (family-M E tables)
 };
 
 enum EpiMode : int {

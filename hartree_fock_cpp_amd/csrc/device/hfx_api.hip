@@ -126,6 +126,10 @@ struct hfx_basis {
     PairHdr* d_hdr = nullptr;
     PrimPair<double>* d_ppd = nullptr;
     PrimPair<float>* d_ppf = nullptr;
+    double* d_etd = nullptr;         // Hermite -> Cartesian matrices of the primitive pairs (family M),
+    float* d_etf = nullptr;          // bra form and (etk) ket form
+    double* d_etkd = nullptr;
+    float* d_etkf = nullptr;
     double* d_fscale = nullptr;
     double* d_bounds = nullptr;      // function-pair Schwarz bounds (P)
     int2* d_fnpairs = nullptr;       // function pair id -> (i, j)
@@ -299,6 +303,8 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
                 L.prim = p ? (const void*)b->d_ppf : (const void*)b->d_ppd;
                 L.boys = p ? (const void*)g_consts.boys_f : (const void*)g_consts.boys_d;
                 L.herm = g_consts.herm;
+                L.etab = p ? (const void*)b->d_etf : (const void*)b->d_etd;
+                L.etabk = p ? (const void*)b->d_etkf : (const void*)b->d_etkd;
                 L.G = GROUP_SIZE[g];
                 L.ep = ep;
                 HIPCHK(launch_eri_class(qc, p == 1, L, ls));
@@ -565,6 +571,10 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
     rc = dalloc(&b->d_hdr, p.hdr.size());
     if (!rc) rc = dalloc(&b->d_ppd, std::max<size_t>(1, p.ppd.size()));
     if (!rc) rc = dalloc(&b->d_ppf, std::max<size_t>(1, p.ppf.size()));
+    if (!rc) rc = dalloc(&b->d_etd, std::max<size_t>(1, p.etd.size()));
+    if (!rc) rc = dalloc(&b->d_etf, std::max<size_t>(1, p.etf.size()));
+    if (!rc) rc = dalloc(&b->d_etkd, std::max<size_t>(1, p.etkd.size()));
+    if (!rc) rc = dalloc(&b->d_etkf, std::max<size_t>(1, p.etkf.size()));
     if (!rc) rc = dalloc(&b->d_fscale, (size_t)K);
     if (!rc) rc = dalloc(&b->d_bounds, (size_t)P);
     if (!rc) rc = dalloc(&b->d_fnpairs, (size_t)P);
@@ -584,6 +594,10 @@ int hfx_basis_create(int K, const double* xyz, const int* lmn, const int* poff, 
     h2d(b->d_hdr, p.hdr.data(), p.hdr.size() * sizeof(PairHdr));
     h2d(b->d_ppd, p.ppd.data(), p.ppd.size() * sizeof(PrimPair<double>));
     h2d(b->d_ppf, p.ppf.data(), p.ppf.size() * sizeof(PrimPair<float>));
+    h2d(b->d_etd, p.etd.data(), p.etd.size() * sizeof(double));
+    h2d(b->d_etf, p.etf.data(), p.etf.size() * sizeof(float));
+    h2d(b->d_etkd, p.etkd.data(), p.etkd.size() * sizeof(double));
+    h2d(b->d_etkf, p.etkf.data(), p.etkf.size() * sizeof(float));
     h2d(b->d_fscale, p.fscale.data(), K * sizeof(double));
     h2d(b->d_fnpairs, p.fnpairs.data(), P * sizeof(int2));
     h2d(b->d_shells, shells.data(), shells.size() * sizeof(int2));
@@ -606,6 +620,10 @@ void hfx_basis_destroy(hfx_basis* b) {
     dfree(b->d_hdr);
     dfree(b->d_ppd);
     dfree(b->d_ppf);
+    dfree(b->d_etd);
+    dfree(b->d_etf);
+    dfree(b->d_etkd);
+    dfree(b->d_etkf);
     dfree(b->d_fscale);
     dfree(b->d_bounds);
     dfree(b->d_fnpairs);

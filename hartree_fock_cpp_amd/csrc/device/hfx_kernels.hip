@@ -40,7 +40,7 @@ hipError_t launch_eri_class(int qc, bool fp32, const ClassLaunch& L, hipStream_t
 
 bool class_uses_family_a(int qc) {
     int pb = qc_bra(qc), pk = qc_ket(qc);
-    return use_family_a(pc_la(pb), pc_lb(pb), pc_la(pk), pc_lb(pk));
+    return runs_family_a(pc_la(pb), pc_lb(pb), pc_la(pk), pc_lb(pk));
 }
 
 // ------------------------------------------------------------------ finish

@@ -13,6 +13,8 @@ struct ClassLaunch {
     const void* prim;      // device PrimPair<Real> array
     const void* boys;      // device Boys table (Real)
     const HermTab* herm;   // device Hermite (t,u,v) decode table
+    const void* etab;      // device Hermite -> Cartesian matrices of the primitive pairs (Real), family M
+    const void* etabk;     // the same, ket form (odd Hermite orders negated)
     int G;                 // family A lanes per quartet (1, 4, 16, 64)
     EpiArgs ep;
 };

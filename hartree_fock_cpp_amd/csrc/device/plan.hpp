@@ -39,7 +39,62 @@ struct BasisPlan {
     std::vector<PrimPair<double>> ppd;
     std::vector<PrimPair<float>> ppf;
     std::vector<int2> fnpairs;          // function pair id -> (i, j), i >= j
+    // Hermite -> Cartesian matrices of every kept primitive pair (PairHdr::eoff):
+    // per primitive pair of a pair of class (la, lb), ncart(la) ncart(lb) rows
+    // (ab = a ncart(lb) + b) of nherm(la + lb) columns (packed tuv), entry
+    // E^x_{a_x b_x, t} E^y_{a_y b_y, u} E^z_{a_z b_z, v} (zero where an
+    // order exceeds a_k + b_k); FP64 and an FP32 copy.  etkd / etkf: the same
+    // with column tuv scaled by (-1)^{t+u+v}, the sign the pair carries on the
+    // ket side of (ab|cd)
+    std::vector<double> etd, etkd;
+    std::vector<float> etf, etkf;
 };
+
+// 1-D Hermite expansion E[a][b][t] of one primitive pair along one axis, for
+// runtime a <= la, b <= lb (the recursion of Herm1D with runtime bounds).
+inline void herm1d_rt(int la, int lb, double i2p, double PA, double PB, double e[4][4][7]) {
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+            for (int t = 0; t < 7; ++t) e[a][b][t] = 0.0;
+    e[0][0][0] = 1.0;
+    for (int a = 1; a <= la; ++a)
+        for (int t = 0; t <= a; ++t) {
+            double v = PA * e[a - 1][0][t];
+            if (t > 0) v += i2p * e[a - 1][0][t - 1];
+            if (t + 1 <= a - 1) v += (t + 1) * e[a - 1][0][t + 1];
+            e[a][0][t] = v;
+        }
+    for (int b = 1; b <= lb; ++b)
+        for (int a = 0; a <= la; ++a)
+            for (int t = 0; t <= a + b; ++t) {
+                double v = (t <= a + b - 1) ? PB * e[a][b - 1][t] : 0.0;
+                if (t > 0) v += i2p * e[a][b - 1][t - 1];
+                if (t + 1 <= a + b - 1) v += (t + 1) * e[a][b - 1][t + 1];
+                e[a][b][t] = v;
+            }
+}
+
+// Appends the Hermite -> Cartesian matrix of one primitive pair (layout in BasisPlan::etd).
+inline void append_emat(int la, int lb, const PrimPair<double>& d, std::vector<double>& etd,
+                        std::vector<double>& etkd) {
+    double ex[3][4][4][7];
+    for (int x = 0; x < 3; ++x) herm1d_rt(la, lb, d.i2p, d.PA[x], d.PB[x], ex[x]);
+    const int L = la + lb;
+    for (int a = 0; a < ncart(la); ++a)
+        for (int b = 0; b < ncart(lb); ++b) {
+            const int ax = cart_x(la, a), ay = cart_y(la, a), az = cart_z(la, a);
+            const int bx = cart_x(lb, b), by = cart_y(lb, b), bz = cart_z(lb, b);
+            for (int s = 0; s <= L; ++s)
+                for (int t = 0; t <= s; ++t)
+                    for (int u = 0; u <= s - t; ++u) {
+                        const int v = s - t - u;
+                        const bool nz = t <= ax + bx && u <= ay + by && v <= az + bz;
+                        const double x = nz ? ex[0][ax][bx][t] * ex[1][ay][by][u] * ex[2][az][bz][v] : 0.0;
+                        etd.push_back(x);
+                        etkd.push_back((s & 1) ? -x : x);
+                    }
+        }
+}
 
 // Drop primitive pairs whose Gaussian-product prefactor exp(-mu |AB|^2) is
 // below e^-60 (~9e-27): their contribution to any integral is < 1e-22.
@@ -131,6 +186,7 @@ inline int build_plan(int K, const double* xyz, const int* lmn, const int* poff,
             h.diag = (A == B);
             h.canon = x;
             h.prim_off = (int)bp.ppd.size();
+            h.eoff = (long long)bp.etd.size();
             const double AB2 = (S1.c[0] - S2.c[0]) * (S1.c[0] - S2.c[0]) + (S1.c[1] - S2.c[1]) * (S1.c[1] - S2.c[1]) +
                                (S1.c[2] - S2.c[2]) * (S1.c[2] - S2.c[2]);
             int kept = 0;
@@ -149,6 +205,7 @@ inline int build_plan(int K, const double* xyz, const int* lmn, const int* poff,
                         d.PB[x3] = d.P[x3] - S2.c[x3];
                     }
                     bp.ppd.push_back(d);
+                    append_emat(S1.L, S2.L, d, bp.etd, bp.etkd);
                     PrimPair<float> f;
                     f.p = (float)d.p;
                     f.i2p = (float)d.i2p;
@@ -165,6 +222,8 @@ inline int build_plan(int K, const double* xyz, const int* lmn, const int* poff,
             bp.hdr[x] = h;
             bp.pclass[x] = pair_class(S1.L, S2.L);
         }
+    bp.etf.assign(bp.etd.begin(), bp.etd.end());
+    bp.etkf.assign(bp.etkd.begin(), bp.etkd.end());
     const int P = K * (K + 1) / 2;
     bp.fnpairs.resize(P);
     for (int i = 0; i < K; ++i)
@@ -195,7 +254,7 @@ inline int group_bucket(int w) { return w >= 512 ? 0 : w >= 64 ? 1 : w >= 8 ? 2 
 
 inline bool qc_family_a(int qc) {
     const int pb = qc_bra(qc), pk = qc_ket(qc);
-    return use_family_a(pc_la(pb), pc_lb(pb), pc_la(pk), pc_lb(pk));
+    return runs_family_a(pc_la(pb), pc_lb(pb), pc_la(pk), pc_lb(pk));
 }
 
 struct QuartetLists {

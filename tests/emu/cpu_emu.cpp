@@ -69,7 +69,7 @@ std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q,
 template <int QC, class Real>
 void run_class(const EmuCtx& c, const std::vector<uint2>& list, const int* bucket, const EpiArgs& ep) {
     using C = QClass<QC>;
-    constexpr bool fam_a = use_family_a(C::LA, C::LB, C::LC, C::LD);
+    constexpr bool fam_a = runs_family_a(C::LA, C::LB, C::LC, C::LD);
     const PrimPair<Real>* pp;
     const Real* tab;
     if constexpr (sizeof(Real) == 8) {
