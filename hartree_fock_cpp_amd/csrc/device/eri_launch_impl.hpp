@@ -64,24 +64,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(famb_waves_p
 }
 
 // One wavefront per work unit, grid-strided; class tables built once per workgroup.
-template <int LA, int LB, int LC, int LD, bool JK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fam_m(const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr,
-                                               const PrimPair<double>* __restrict__ pp,
-                                               const double* __restrict__ etab, const double* __restrict__ etabk,
-                                               const double* __restrict__ tab,
-                                               const HermTab* __restrict__ ht, EpiArgs ep) {
+template <int LA, int LB, int LC, int LD, class Real, bool JK>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fam_m(
+    const uint2* __restrict__ list, int n, const PairHdr* __restrict__ hdr, const PrimPair<double>* __restrict__ ppd,
+    const Real* __restrict__ etab, const Real* __restrict__ etabk, const double* __restrict__ tabd,
+    const HermTab* __restrict__ ht, EpiArgs ep) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    __shared__ FamMSmem<LA, LB, LC, LD, double, JK> sm;
+    __shared__ FamMSmem<LA, LB, LC, LD, Real, JK> sm;
     famm_tables<LA, LB, LC, LD>(sm, *ht, (int)threadIdx.x);
     __syncthreads();
     for (long long u = blockIdx.x; u < n; u += gridDim.x)
-        family_m_one<LA, LB, LC, LD, double, JK>(list[u], hdr, pp, etab, etabk, tab, ep, sm, (int)threadIdx.x);
+        family_m_one<LA, LB, LC, LD, Real, JK>(list[u], hdr, ppd, etab, etabk, tabd, ep, sm, (int)threadIdx.x);
 #endif
 }
 
-// Family M (MFMA contractions) for the FP64 kernels of the classes that
-// use_family_m admits; HFX_FAMM=0 in the environment selects family B
-// instead (measurement).
+// Family M (MFMA contractions) for the classes that use_family_m admits, in
+// both precisions; HFX_FAMM=0 in the environment selects family B instead
+// (measurement).
 inline bool famm_enabled() {
     static const bool on = [] {
         const char* e = std::getenv("HFX_FAMM");
@@ -106,12 +105,13 @@ hipError_t launch_class_mode(const ClassLaunch& L, hipStream_t st) {
             default: return hipErrorInvalidValue;
         }
     } else {
-        if constexpr (std::is_same_v<Real, double> && use_family_m(C::LA, C::LB, C::LC, C::LD)) {
+        if constexpr (use_family_m(C::LA, C::LB, C::LC, C::LD)) {
             if (famm_enabled()) {
                 const int grid = L.n < 65536 ? L.n : 65536;
-                k_fam_m<C::LA, C::LB, C::LC, C::LD, JK><<<grid, 64, 0, st>>>(
-                    L.list, L.n, L.hdr, pp, static_cast<const double*>(L.etab), static_cast<const double*>(L.etabk), tab,
-                    L.herm, L.ep);
+                k_fam_m<C::LA, C::LB, C::LC, C::LD, Real, JK><<<grid, 64, 0, st>>>(
+                    L.list, L.n, L.hdr, static_cast<const PrimPair<double>*>(L.prim64),
+                    static_cast<const Real*>(L.etab), static_cast<const Real*>(L.etabk),
+                    static_cast<const double*>(L.boys64), L.herm, L.ep);
                 return hipGetLastError();
             }
         }

@@ -24,6 +24,15 @@
 // MFMA lane layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16]
 // and holds C[l/16 + 4r][l%16], r = 0..3.
 //
+// FP32 form (the mixed-precision path's small-bound quartets): E matrices,
+// R recursion, both GEMMs and their accumulation in FP32 (v_mfma_f32_16x16x4f32,
+// whose accumulator layout, lane l holding C[4(l/16) + r][l%16], is the
+// transpose of the FP64 one across lane groups, so G passes from stage 1 to
+// stage 2 through LDS), while the Boys function, the prefactors and P - Q are
+// evaluated in FP64 from the FP64 primitive pairs and rounded once (the FP64
+// islands: the Boys function at any T and n, and the E matrices, which are
+// computed in FP64 on the host).
+//
 // One wavefront per work unit (a shell quartet or a primitive slice of one,
 // famb_range), bra primitive pairs in groups of PG (stage-1 rows PG x NHAB),
 // ket primitive pairs in passes of QC (their R arrays in LDS), ket
@@ -92,6 +101,7 @@ struct FamMSmem {
     Real a[S::NCUBE][S::L + 1];        // pref (-2 rho)^n F_n(T)
     Real x[S::NCUBE][3];               // P - Q
     Real acc[Bs::NAB * S::CDM];        // a chunk's block values for the epilogue
+    Real gt[sizeof(Real) == 4 ? 16 * S::MT1 * S::CDM : 1];   // FP32: G in [(p,tuv)][cd] layout for stage 2
     double epi[JK ? Bs::NEPI : Bs::O_DJAB];
     int rsrc[S::NR];                   // recursion sources: i1 | i2 << 10 | coef << 20 | axis << 26 (axis 3: base)
     unsigned short lut[S::NHAB * S::NHCD];   // hidx(t+tau, u+nu, v+phi)
@@ -135,11 +145,13 @@ __device__ __forceinline__ int famm_off(const SM& sm, int m, int kb, int lane) {
 // One work unit on one wavefront (lane = 0..63, all active).  etab / etabk:
 // the E tables in bra and ket form (BasisPlan::etd / etkd, or the FP32
 // copies) in the precision of the class kernel.
+// pp / tab: the FP64 primitive pairs and Boys table (both precisions
+// evaluate the Boys function and prefactors in FP64).
 template <int LA, int LB, int LC, int LD, class Real, bool JK>
-__device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const PrimPair<Real>* __restrict__ pp,
+__device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const PrimPair<double>* __restrict__ pp,
                              const Real* __restrict__ etab, const Real* __restrict__ etabk,
-                             const Real* __restrict__ tab, const EpiArgs& ep, FamMSmem<LA, LB, LC, LD, Real, JK>& sm,
-                             int lane) {
+                             const double* __restrict__ tab, const EpiArgs& ep,
+                             FamMSmem<LA, LB, LC, LD, Real, JK>& sm, int lane) {
     using S = FamMShape<LA, LB, LC, LD>;
     using V4 = typename MfmaOps<Real>::V4;
     auto sync = []() { __syncthreads(); };
@@ -183,25 +195,25 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
                 for (int c = lane; c < nqc * S::PG; c += 64) {
                     const int pi = c % S::PG, qi = c / S::PG;
                     if (pi >= npg) continue;
-                    const PrimPair<Real> b = pp[hb.prim_off + rg.i0 + (pg0 + pi) * rg.istep];
-                    const PrimPair<Real> k = pp[hk.prim_off + q0 + qi];
-                    const Real s = b.p + k.p;
-                    const Real inv = Real(1) / s;
-                    const Real rho = b.p * k.p * inv;
-                    const Real X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
-                    const Real T = rho * (X * X + Y * Y + Z * Z);
-                    Real F[S::L + 1];
-                    boys_fm<S::L, Real>(T, tab, F);
-                    Real f = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
-                    const Real m2r = Real(-2) * rho;
+                    const PrimPair<double> b = pp[hb.prim_off + rg.i0 + (pg0 + pi) * rg.istep];
+                    const PrimPair<double> k = pp[hk.prim_off + q0 + qi];
+                    const double s = b.p + k.p;
+                    const double inv = 1.0 / s;
+                    const double rho = b.p * k.p * inv;
+                    const double X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
+                    const double T = rho * (X * X + Y * Y + Z * Z);
+                    double F[S::L + 1];
+                    boys_fm<S::L, double>(T, tab, F);
+                    double f = TWO_PI_POW_2_5 * b.w * k.w * sqrt(inv);
+                    const double m2r = -2.0 * rho;
 #pragma unroll
                     for (int n = 0; n <= S::L; ++n) {
-                        sm.a[c][n] = f * F[n];
+                        sm.a[c][n] = (Real)(f * F[n]);
                         f *= m2r;
                     }
-                    sm.x[c][0] = X;
-                    sm.x[c][1] = Y;
-                    sm.x[c][2] = Z;
+                    sm.x[c][0] = (Real)X;
+                    sm.x[c][1] = (Real)Y;
+                    sm.x[c][2] = (Real)Z;
                     sm.cube[c][0] = sm.a[c][S::L];
                 }
                 sync();
@@ -254,6 +266,15 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
                 }
             }
             // stage 2: acc[ab][cd] += sum_{(p,tuv)} Eb[ab][(p,tuv)] G[(p,tuv)][cd]
+            if constexpr (sizeof(Real) == 4) {   // FP32 layout: G through LDS, [(p,tuv)][cd]
+#pragma unroll
+                for (int m = 0; m < S::MT1; ++m)
+#pragma unroll
+                    for (int n = 0; n < S::NT; ++n)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) sm.gt[(16 * m + 4 * g + r) * S::CDM + 16 * n + c16] = G[m][n][r];
+                sync();
+            }
 #pragma unroll
             for (int kk = 0; kk < S::KT2; ++kk) {
                 const int krow = 4 * kk + g;   // this lane's (p, tuv) of the A operand
@@ -265,18 +286,26 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
                     const int ab = 16 * m + c16;
                     const Real av = (kin && ab < S::NAB) ? ebp[ab * S::NHAB + tuv] : Real(0);
 #pragma unroll
-                    for (int n = 0; n < S::NT; ++n) acc[m][n] = MfmaOps<Real>::mma(av, G[kk / 4][n][kk % 4], acc[m][n]);
+                    for (int n = 0; n < S::NT; ++n) {
+                        Real bv;
+                        if constexpr (sizeof(Real) == 4)
+                            bv = sm.gt[krow * S::CDM + 16 * n + c16];
+                        else
+                            bv = G[kk / 4][n][kk % 4];
+                        acc[m][n] = MfmaOps<Real>::mma(av, bv, acc[m][n]);
+                    }
                 }
             }
         }
-        // the chunk's block into LDS (lane holds acc[ab = 16m + g + 4r][cd = 16n + c16])
+        // the chunk's block into LDS (lane holds acc[ab = 16m + g + 4r][cd = 16n + c16]
+        // in FP64, acc[ab = 16m + 4g + r][cd] in FP32)
 #pragma unroll
         for (int m = 0; m < S::MT2; ++m)
 #pragma unroll
             for (int n = 0; n < S::NT; ++n)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int ab = 16 * m + g + 4 * r, cdl = 16 * n + c16;
+                    const int ab = sizeof(Real) == 4 ? 16 * m + 4 * g + r : 16 * m + g + 4 * r, cdl = 16 * n + c16;
                     if (ab < S::NAB && cdl < ncd) sm.acc[ab * S::CDM + cdl] = acc[m][n][r];
                 }
         sync();

@@ -305,6 +305,8 @@ int launch_plan(const hfx_basis* b, const DevLists& dl, const EpiArgs& ep, hipSt
                 L.herm = g_consts.herm;
                 L.etab = p ? (const void*)b->d_etf : (const void*)b->d_etd;
                 L.etabk = p ? (const void*)b->d_etkf : (const void*)b->d_etkd;
+                L.prim64 = b->d_ppd;
+                L.boys64 = g_consts.boys_d;
                 L.G = GROUP_SIZE[g];
                 L.ep = ep;
                 HIPCHK(launch_eri_class(qc, p == 1, L, ls));

@@ -15,6 +15,8 @@ struct ClassLaunch {
     const HermTab* herm;   // device Hermite (t,u,v) decode table
     const void* etab;      // device Hermite -> Cartesian matrices of the primitive pairs (Real), family M
     const void* etabk;     // the same, ket form (odd Hermite orders negated)
+    const void* prim64;    // FP64 primitive pairs and Boys table (family M evaluates Boys in FP64 in both precisions)
+    const void* boys64;
     int G;                 // family A lanes per quartet (1, 4, 16, 64)
     EpiArgs ep;
 };

@@ -126,8 +126,7 @@ def model_flops(a, b, c, d):
 
 def kernel_family(qc, fp32=False):
     """Name of the kernel template that runs class qc (mirrors use_family_a,
-    use_family_m and runs_family_a in csrc/device/eri_body.hpp; family M runs
-    the FP64 lists, family B the FP32 lists of its classes)."""
+    use_family_m and runs_family_a in csrc/device/eri_body.hpp)."""
     a, b, c, d = qclass_l(qc)
     L = a + b + c + d
     nout = len(_cart(a)) * len(_cart(b)) * len(_cart(c)) * len(_cart(d))
@@ -135,5 +134,5 @@ def kernel_family(qc, fp32=False):
     ncd = len(_cart(c)) * len(_cart(d))
     fam_m = math.comb(c + d + 3, 3) >= 4 and ncd >= 6 and (not fam_a or (a, b, c, d) == (1, 1, 1, 1))
     if fam_m:
-        return "k_fam_b" if fp32 else "k_fam_m"
+        return "k_fam_m"
     return "k_fam_a" if fam_a else "k_fam_b"
