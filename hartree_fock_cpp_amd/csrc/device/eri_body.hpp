@@ -279,87 +279,134 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
         return;
     }
     } else {
-    // EPI_JK
+    // EPI_JK.  Every operand is loaded before the first atomic (the
+    // compiler cannot move a load above an atomic that may alias it, so
+    // interleaving them would serialise one memory latency per term), then
+    // the block is scaled and masked once, the J and K sums formed in
+    // registers, and the atomics issued back to back.
     const int K = ep.K;
     const double f = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
-    double v[NA * NB * NCD];
+    constexpr int NAB = NA * NB;
+    double fsa[NA], fsb[NB], fsc[NC], fsd[ND];
+    double bab[NAB], bcd[NCD], djab[NAB], djcd[NCD];
+    double dac[NA * NC], dad[NA * ND], dbc[NB * NC], dbd[NB * ND];
 #pragma unroll
-    for (int e = 0; e < NA * NB * NCD; ++e) {
-        const int a = e / (NB * NCD), b = (e / NCD) % NB, c = (e / ND) % NC, d = e % ND;
-        const int mu = q.fn1 + a, nu = q.fn2 + b, la = q.fn3 + c, si = q.fn4 + d;
-        v[e] = mask_pass(ep, mu, nu, la, si) ? (double)acc[e] * (fs[mu] * fs[nu] * fs[la] * fs[si]) : 0.0;
-    }
-    // J
+    for (int a = 0; a < NA; ++a) fsa[a] = fs[q.fn1 + a];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) fsb[b] = fs[q.fn2 + b];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) fsc[c] = fs[q.fn3 + c];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) fsd[d] = fs[q.fn4 + d];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int d = 0; d < ND; ++d) s += v[(a * NB + b) * NCD + c * ND + d] * ep.Dj[(q.fn3 + c) * K + q.fn4 + d];
-            atomic_add(&ep.J[(q.fn1 + a) * K + q.fn2 + b], 2.0 * f * s);
+            bab[a * NB + b] = ep.bounds[pair_of(q.fn1 + a, q.fn2 + b)];
+            djab[a * NB + b] = ep.Dj[(q.fn1 + a) * K + q.fn2 + b];
         }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            double s = 0.0;
-#pragma unroll
-            for (int a = 0; a < NA; ++a)
-#pragma unroll
-                for (int b = 0; b < NB; ++b) s += v[(a * NB + b) * NCD + c * ND + d] * ep.Dj[(q.fn1 + a) * K + q.fn2 + b];
-            atomic_add(&ep.J[(q.fn3 + c) * K + q.fn4 + d], 2.0 * f * s);
+            bcd[c * ND + d] = ep.bounds[pair_of(q.fn3 + c, q.fn4 + d)];
+            djcd[c * ND + d] = ep.Dj[(q.fn3 + c) * K + q.fn4 + d];
         }
-    // K (one or two spins)
-    for (int spin = 0; spin < ep.nspin; ++spin) {
-        const double* D = spin == 0 ? ep.Da : ep.Db;
-        double* Kacc = spin == 0 ? ep.Ka : ep.Kb;
+    auto load_k = [&](const double* D) {
 #pragma unroll
-        for (int a = 0; a < NA; ++a)
+        for (int a = 0; a < NA; ++a) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                double s = 0.0;   // Kacc_ac += f sum_bd v D_bd
+            for (int c = 0; c < NC; ++c) dac[a * NC + c] = D[(q.fn1 + a) * K + q.fn3 + c];
 #pragma unroll
-                for (int b = 0; b < NB; ++b)
+            for (int d = 0; d < ND; ++d) dad[a * ND + d] = D[(q.fn1 + a) * K + q.fn4 + d];
+        }
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn2 + b) * K + q.fn4 + d];
-                atomic_add(&Kacc[(q.fn1 + a) * K + q.fn3 + c], f * s);
-            }
+        for (int b = 0; b < NB; ++b) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+            for (int c = 0; c < NC; ++c) dbc[b * NC + c] = D[(q.fn2 + b) * K + q.fn3 + c];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                double s = 0.0;   // Kacc_bc += f sum_ad v D_ad
+            for (int d = 0; d < ND; ++d) dbd[b * ND + d] = D[(q.fn2 + b) * K + q.fn4 + d];
+        }
+    };
+    load_k(ep.Da);
+    double v[NA * NB * NCD];
 #pragma unroll
-                for (int a = 0; a < NA; ++a)
-#pragma unroll
-                    for (int d = 0; d < ND; ++d) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn1 + a) * K + q.fn4 + d];
-                atomic_add(&Kacc[(q.fn2 + b) * K + q.fn3 + c], f * s);
-            }
-#pragma unroll
-        for (int a = 0; a < NA; ++a)
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                double s = 0.0;   // Kacc_ad += f sum_bc v D_bc
-#pragma unroll
-                for (int b = 0; b < NB; ++b)
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn2 + b) * K + q.fn3 + c];
-                atomic_add(&Kacc[(q.fn1 + a) * K + q.fn4 + d], f * s);
-            }
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                double s = 0.0;   // Kacc_bd += f sum_ac v D_ac
+            for (int c = 0; c < NC; ++c)
 #pragma unroll
-                for (int a = 0; a < NA; ++a)
+                for (int d = 0; d < ND; ++d) {
+                    const int e = (a * NB + b) * NCD + c * ND + d;
+                    v[e] = bab[a * NB + b] * bcd[c * ND + d] >= ep.threshold
+                               ? (double)acc[e] * (fsa[a] * fsb[b] * fsc[c] * fsd[d]) : 0.0;
+                }
+    double jab[NAB], jcd[NCD];
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) s += v[(a * NB + b) * NCD + c * ND + d] * D[(q.fn1 + a) * K + q.fn3 + c];
-                atomic_add(&Kacc[(q.fn2 + b) * K + q.fn4 + d], f * s);
-            }
+    for (int x = 0; x < NAB; ++x) jab[x] = 0.0;
+#pragma unroll
+    for (int y = 0; y < NCD; ++y) jcd[y] = 0.0;
+#pragma unroll
+    for (int x = 0; x < NAB; ++x)
+#pragma unroll
+        for (int y = 0; y < NCD; ++y) {
+            jab[x] += v[x * NCD + y] * djcd[y];
+            jcd[y] += v[x * NCD + y] * djab[x];
+        }
+    // K sums of one spin: Kacc_ac += f sum_bd v D_bd, Kacc_bc += f sum_ad v D_ad,
+    // Kacc_ad += f sum_bc v D_bc, Kacc_bd += f sum_ac v D_ac
+    auto k_terms = [&](double* Kacc) {
+        double kac[NA * NC], kbc[NB * NC], kad[NA * ND], kbd[NB * ND];
+#pragma unroll
+        for (int x = 0; x < NA * NC; ++x) kac[x] = 0.0;
+#pragma unroll
+        for (int x = 0; x < NB * NC; ++x) kbc[x] = 0.0;
+#pragma unroll
+        for (int x = 0; x < NA * ND; ++x) kad[x] = 0.0;
+#pragma unroll
+        for (int x = 0; x < NB * ND; ++x) kbd[x] = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) {
+                        const double x = v[(a * NB + b) * NCD + c * ND + d];
+                        kac[a * NC + c] += x * dbd[b * ND + d];
+                        kbc[b * NC + c] += x * dad[a * ND + d];
+                        kad[a * ND + d] += x * dbc[b * NC + c];
+                        kbd[b * ND + d] += x * dac[a * NC + c];
+                    }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) atomic_add(&Kacc[(q.fn1 + a) * K + q.fn3 + c], f * kac[a * NC + c]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) atomic_add(&Kacc[(q.fn1 + a) * K + q.fn4 + d], f * kad[a * ND + d]);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) atomic_add(&Kacc[(q.fn2 + b) * K + q.fn3 + c], f * kbc[b * NC + c]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) atomic_add(&Kacc[(q.fn2 + b) * K + q.fn4 + d], f * kbd[b * ND + d]);
+        }
+    };
+    k_terms(ep.Ka);
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) atomic_add(&ep.J[(q.fn1 + a) * K + q.fn2 + b], 2.0 * f * jab[a * NB + b]);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) atomic_add(&ep.J[(q.fn3 + c) * K + q.fn4 + d], 2.0 * f * jcd[c * ND + d]);
+    if (ep.nspin == 2) {
+        load_k(ep.Db);
+        k_terms(ep.Kb);
     }
     }
 }
