@@ -220,11 +220,83 @@ def stored_jk(hfx, N, basis, thr, steps):
             "traffic": traffic, "timing": f"HIP events on the launch stream over {steps} builds"}
 
 
-def pcie_inclusive(hfx, N, basis, thr, q_screened, reps=5):
+def direct_build(name, atoms, basis_set, steps, thr=1e-12, split=1e-5):
+    """Full (non-incremental) direct J/K builds, FP64 and mixed, on device
+    buffers at a fixed density: per-build time from HIP events on the launch
+    stream, model GFLOP per build (screened primitive quartets x per-class
+    flops), and the dominant class kernel from a serialized replay."""
+    import numpy as np
+    import torch
+    import hartree_fock_cpp_amd as hfx
+    from hartree_fock_cpp_amd import _native as N
+    from hartree_fock_cpp_amd import workloads as W
+    from hartree_fock_cpp_amd.basis import basis_construction
+    b = hfx.Basis.from_arrays(basis_construction(atoms, basis_set))
+    K = b.K
+    g = torch.Generator().manual_seed(0)
+    A = torch.randn(K, K, generator=g, dtype=torch.float64)
+    P = ((A @ A.T) / K).cuda()
+    J = torch.empty(K, K, dtype=torch.float64, device="cuda")
+    Kx = torch.empty_like(J)
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    sh = C.c_void_p(st.cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    out = {"workload": f"{name} RHF direct J/K build (full, not incremental), Schwarz {thr:g}", "K": K}
+    for prec in ("fp64", "mixed"):
+        pcode = N.HFX_PREC_MIXED if prec == "mixed" else N.HFX_PREC_FP64
+        nq = np.zeros(110, np.int64)
+        nprimq = np.zeros(110, np.int64)
+        N.check(N.lib.hfx_plan_stats(b.handle(), C.c_double(thr), pcode, C.c_double(split), N.ptr(nq),
+                                     N.ptr(nprimq)))
+        fl = np.array([W.model_flops(*W.qclass_l(i // 2)) for i in range(110)], float) * nprimq
+        fb = hfx.FockBuilder(b, mode="direct", precision=prec, threshold=thr, mixed_split=split)
+        build = lambda: N.check(N.lib.hfx_fock_build_device(fb._h, ptr(P), None, ptr(J), ptr(Kx), None, sh))
+        for _ in range(2):
+            build()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(steps):
+            build()
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        N.check(N.lib.hfx_set_class_timing(2))
+        cls = np.zeros(110)
+        buf = np.zeros(110, np.float32)
+        for _ in range(steps):
+            build()
+            st.synchronize()
+            N.check(N.lib.hfx_class_times(N.ptr(buf)))
+            cls += buf
+        N.check(N.lib.hfx_set_class_timing(0))
+        torch.cuda.synchronize()
+        fb.close()
+        cls /= steps
+        dom = int(np.argmax(cls))
+        r = {"ms_per_build": ms, "model_gflop": fl.sum() / 1e9, "tflops": fl.sum() / (ms * 1e-3) / 1e12,
+             "frac_fp64_peak": fl.sum() / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+             "shell_quartets": int(nq.sum()), "fp32_shell_quartets": int(nq[1::2].sum()),
+             "serialized_class_ms": float(cls.sum()),
+             "dominant": {"kernel": "%s %s fp%s" % (W.kernel_family(dom // 2, dom % 2 == 1), W.qclass_name(dom // 2),
+                                                    "32" if dom % 2 else "64"),
+                          "launch_ms": float(cls[dom]), "model_gflop": fl[dom] / 1e9,
+                          "tflops": fl[dom] / (cls[dom] * 1e-3) / 1e12 if cls[dom] > 0 else None}}
+        out[prec] = r
+        log(f"  {name} direct build {prec}: {ms:.2f} ms, {r['tflops']:.2f} model TFLOP/s, "
+            f"dominant {r['dominant']['kernel']} {cls[dom]:.3f} ms")
+    out["mixed_over_fp64"] = out["fp64"]["ms_per_build"] / out["mixed"]["ms_per_build"]
+    b.close()
+    return out
+
+
+def pcie_inclusive(hfx, N, basis, arrs, thr, q_screened, reps=5):
     """The drop-in as the reference calls it: hfx_eri_unique into a pageable
     host array (the unique build plus the device-to-host copy of the whole
     Q-long array), wall-clock per call.  Reported beside the headline, never
-    as its value (the headline keeps the array resident in HBM)."""
+    as its value (the headline keeps the array resident in HBM).  "cold" is
+    what one ERI_GPU call through the shim costs: a fresh basis handle (device
+    tables, Schwarz bounds, quartet plan), the build, the copy, the teardown."""
     import numpy as np
     Q = basis.n_quartets
     out = np.empty(Q)
@@ -236,9 +308,25 @@ def pcie_inclusive(hfx, N, basis, thr, q_screened, reps=5):
         call()
         ts.append(time.perf_counter() - t)
     ms = float(np.median(ts)) * 1e3
-    log(f"  PCIe-inclusive unique build: {ms:.2f} ms per call ({8 * Q / 1e6:.0f} MB to the host)")
+
+    def cold():
+        b = hfx.Basis.from_arrays(arrs)
+        N.check(N.lib.hfx_eri_unique(b.handle(), C.c_double(thr), N.ptr(out), C.c_size_t(Q)))
+        b.close()
+    cold()
+    tc = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        cold()
+        tc.append(time.perf_counter() - t)
+    ms_cold = float(np.median(tc)) * 1e3
+    log(f"  PCIe-inclusive unique build: {ms:.2f} ms per call, cold {ms_cold:.2f} ms "
+        f"({8 * Q / 1e6:.0f} MB to the host)")
     return {"ms_per_call": ms, "value": q_screened / (ms * 1e-3), "unit": "quartets/s",
-            "bytes_to_host": 8 * Q, "timing": f"median wall clock of {reps} calls into a pageable numpy array"}
+            "bytes_to_host": 8 * Q, "timing": f"median wall clock of {reps} calls into a pageable numpy array",
+            "cold_ms_per_call": ms_cold, "cold_value": q_screened / (ms_cold * 1e-3),
+            "cold_timing": f"median of {reps} calls, each: basis handle from the basis arrays, Schwarz bounds, "
+                           "quartet plan, build, copy to the host array, handle destroyed"}
 
 
 def stored_scf(name, atoms, basis_set, thr=1e-8):
@@ -310,6 +398,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the SCF wall-time, mixed-vs-FP64 and serving sections")
+    ap.add_argument("--direct-build", choices=["water4", "octane"], default=None,
+                    help="only time full direct J/K builds (FP64 and mixed) of (H2O)4/cc-pVTZ or "
+                         "n-octane/cc-pVDZ and print their JSON object")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
                          "for scaling HBM byte counters) and exit")
@@ -323,6 +414,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": 5}), flush=True)
+        return
+    if args.direct_build:
+        from hartree_fock_cpp_amd import workloads as W
+        mol = {"water4": ("(H2O)4/cc-pVTZ", W.water_tetramer, "cc-pVTZ"),
+               "octane": ("n-octane/cc-pVDZ", W.n_octane, "cc-pVDZ")}[args.direct_build]
+        print(json.dumps(direct_build(mol[0], mol[1](), mol[2], args.steps)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one MI355X: the engine runs on a single GPU; sharding over several GPUs "
@@ -385,6 +482,7 @@ def main():
     dom = int(np.argmax(cls_ms))
     dom_ms = cls_ms[dom] / args.steps
     dom_tflops = flops_cls[dom] / (dom_ms * 1e-3) / 1e12
+    fam_dom = W.kernel_family(dom // 2, dom % 2 == 1)
     build_tflops = flops_cls.sum() / (ms_per_step * 1e-3) / 1e12
 
     traffic = None
@@ -416,8 +514,10 @@ def main():
                    "K": basis.K, "Q_total": Q, "Q_screened": q_screened, "threshold": thr,
                    "shell_quartets": int(nq.sum()), "primitive_quartets": int(nprimq.sum()),
                    "model_gflop_per_build": flops_cls.sum() / 1e9},
-        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU; same 78.6 TFLOP/s peak as FP64 MFMA)",
-                     "kernel": "%s %s fp%s" % (W.kernel_family(dom // 2, dom % 2 == 1), W.qclass_name(dom // 2),
+        "roofline": {"bound": "mfma" if fam_dom == "k_fam_m" else "valu",
+                     "pipe": "fp64 %s (FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak)"
+                             % ("MFMA" if fam_dom == "k_fam_m" else "VALU"),
+                     "kernel": "%s %s fp%s" % (fam_dom, W.qclass_name(dom // 2),
                                                "32" if dom % 2 else "64"),
                      "achieved": dom_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": dom_tflops / PEAK_FP64_TFLOPS, "traffic": traffic,
@@ -430,7 +530,7 @@ def main():
     if cpu:
         line["gpu_over_cpu"] = line["value"] / cpu["value"]
     line["stored_jk"] = stored_jk(hfx, N, basis, thr, args.steps)
-    line["pcie_inclusive"] = pcie_inclusive(hfx, N, basis, thr, q_screened)
+    line["pcie_inclusive"] = pcie_inclusive(hfx, N, basis, arrs, thr, q_screened)
     if not args.no_extras:
         log("extras: SCF wall times")
         scf = {"benzene_ccpvdz_stored_fp64": stored_scf("benzene/cc-pVDZ", atoms, "cc-pVDZ")}
@@ -439,6 +539,10 @@ def main():
         log("extras: mixed vs FP64 direct SCF")
         scf["octane_ccpvdz_direct"] = direct_scf_pair("n-octane/cc-pVDZ", W.n_octane(), "cc-pVDZ")
         scf["water4_ccpvtz_direct"] = direct_scf_pair("(H2O)4/cc-pVTZ", W.water_tetramer(), "cc-pVTZ")
+        line["direct_build"] = {"water4_ccpvtz": direct_build("(H2O)4/cc-pVTZ", W.water_tetramer(), "cc-pVTZ",
+                                                              min(args.steps, 5)),
+                                "octane_ccpvdz": direct_build("n-octane/cc-pVDZ", W.n_octane(), "cc-pVDZ",
+                                                              min(args.steps, 5))}
         scf["o2_ccpvtz_uhf_triplet"] = uhf_scf("O2/cc-pVTZ UHF triplet", W.o2(), "cc-pVTZ", (9, 7))
         line["scf"] = scf
         log("extras: published-table molecules")

@@ -89,7 +89,7 @@ struct FamMShape {
     static constexpr int QC = QC0 < 1 ? 1 : QC0 > 64 ? 64 : QC0;
     static constexpr int NCUBE = QC * PG;
     // stage-1 gather offsets held per lane in registers when few
-    static constexpr bool OFFREG = MT1 * KT1 <= 40;
+    static constexpr bool OFFREG = MT1 * KT1 <= 12;
 };
 
 
@@ -194,7 +194,13 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
                 // Boys function and prefactors of each (p, q): one lane each
                 for (int c = lane; c < nqc * S::PG; c += 64) {
                     const int pi = c % S::PG, qi = c / S::PG;
-                    if (pi >= npg) continue;
+                    if (pi >= npg) {   // absent bra pair: zero arrays, so stage 1 reads them unmasked
+#pragma unroll
+                        for (int n = 0; n <= S::L; ++n) sm.a[c][n] = Real(0);
+                        sm.x[c][0] = sm.x[c][1] = sm.x[c][2] = Real(0);
+                        sm.cube[c][0] = Real(0);
+                        continue;
+                    }
                     const PrimPair<double> b = pp[hb.prim_off + rg.i0 + (pg0 + pi) * rg.istep];
                     const PrimPair<double> k = pp[hk.prim_off + q0 + qi];
                     const double s = b.p + k.p;
@@ -238,19 +244,26 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
                     for (int c = lane; c < ncube; c += 64) sm.cube[c][0] = sm.a[c][n];
                     sync();
                 });
-                // stage 1: G[(p,tuv)][cd] += sum_{tau nu phi} A1 B1, per ket pair of the pass
+                // stage 1: G[(p,tuv)][cd] += sum_{tau nu phi} A1 B1, per ket pair of the pass.
+                // Every load reads a valid address (indices clamped) and is masked
+                // afterwards, so the loads of a k-block issue together.
                 for (int qi = 0; qi < nqc; ++qi) {
-                    const Real* ekq = ek + (long long)(q0 + qi) * (S::NCD * S::NHCD);
-#pragma unroll
-                    for (int kb = 0; kb < S::KT1; ++kb) {
-                        const int tnp = 4 * kb + g;   // this lane's tau nu phi
+                    const Real* ekq = ek + (long long)(q0 + qi) * (S::NCD * S::NHCD);   // [tau nu phi][cd]
+                    const Real* cq = sm.cube[qi * S::PG];
+                    int gl = g, cl = c16;
+                    asm volatile("" : "+v"(gl), "+v"(cl));
+                    static_for<0, S::KT1>([&](auto kbi) {
+                        constexpr int kb = decltype(kbi)::value;
+                        const int tnp = 4 * kb + gl;   // this lane's tau nu phi
+                        const bool tin = tnp < S::NHCD;
+                        const Real* er = ekq + (tin ? tnp : 0) * S::NCD;
                         Real bf[S::NT];
 #pragma unroll
                         for (int n = 0; n < S::NT; ++n) {
-                            const int cd = cd0 + 16 * n + c16;
-                            bf[n] = (cd < S::NCD && tnp < S::NHCD) ? ekq[cd * S::NHCD + tnp] : Real(0);
+                            const int cd = cd0 + 16 * n + cl;
+                            const Real v = er[cd < S::NCD ? cd : S::NCD - 1];
+                            bf[n] = (tin && cd < S::NCD) ? v : Real(0);
                         }
-                        const Real* cq = sm.cube[qi * S::PG];
 #pragma unroll
                         for (int m = 0; m < S::MT1; ++m) {
                             int o;
@@ -258,11 +271,12 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
                                 o = off[m][kb];
                             else
                                 o = famm_off<S>(sm, m, kb, lane);
-                            const Real av = (o >= 0 && o < npg * S::NR) ? cq[o] : Real(0);
+                            const Real v = cq[o >= 0 ? o : 0];
+                            const Real av = o >= 0 ? v : Real(0);
 #pragma unroll
                             for (int n = 0; n < S::NT; ++n) G[m][n] = MfmaOps<Real>::mma(av, bf[n], G[m][n]);
                         }
-                    }
+                    });
                 }
             }
             // stage 2: acc[ab][cd] += sum_{(p,tuv)} Eb[ab][(p,tuv)] G[(p,tuv)][cd]
@@ -275,26 +289,35 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
                         for (int r = 0; r < 4; ++r) sm.gt[(16 * m + 4 * g + r) * S::CDM + 16 * n + c16] = G[m][n][r];
                 sync();
             }
+            {
+                const long long pstr = (long long)rg.istep * (S::NAB * S::NHAB);
+                const Real* ebg = eb + (long long)(rg.i0 + pg0 * rg.istep) * (S::NAB * S::NHAB);   // [tuv][ab]
+                // lane coordinates made opaque here: the per-k-block offsets below are
+                // recomputed in this loop instead of hoisted (and spilled) as invariants
+                int gl = g, cl = c16;
+                asm volatile("" : "+v"(gl), "+v"(cl));
+                static_for<0, S::KT2>([&](auto kki) {
+                    constexpr int kk = decltype(kki)::value;
+                    const int krow = 4 * kk + gl;   // this lane's (p, tuv) of the A operand
+                    const int pi = krow / S::NHAB, tuv = krow - pi * S::NHAB;
+                    const bool kin = pi < npg;
+                    const Real* src = ebg + (kin ? pi : 0) * pstr + tuv * S::NAB;
 #pragma unroll
-            for (int kk = 0; kk < S::KT2; ++kk) {
-                const int krow = 4 * kk + g;   // this lane's (p, tuv) of the A operand
-                const int pi = krow / S::NHAB, tuv = krow - pi * S::NHAB;
-                const bool kin = pi < npg;
-                const Real* ebp = eb + (long long)(rg.i0 + (pg0 + (kin ? pi : 0)) * rg.istep) * (S::NAB * S::NHAB);
+                    for (int m = 0; m < S::MT2; ++m) {
+                        const int ab = 16 * m + cl;
+                        const Real v = src[ab < S::NAB ? ab : S::NAB - 1];
+                        const Real av = (kin && ab < S::NAB) ? v : Real(0);
 #pragma unroll
-                for (int m = 0; m < S::MT2; ++m) {
-                    const int ab = 16 * m + c16;
-                    const Real av = (kin && ab < S::NAB) ? ebp[ab * S::NHAB + tuv] : Real(0);
-#pragma unroll
-                    for (int n = 0; n < S::NT; ++n) {
-                        Real bv;
-                        if constexpr (sizeof(Real) == 4)
-                            bv = sm.gt[krow * S::CDM + 16 * n + c16];
-                        else
-                            bv = G[kk / 4][n][kk % 4];
-                        acc[m][n] = MfmaOps<Real>::mma(av, bv, acc[m][n]);
+                        for (int n = 0; n < S::NT; ++n) {
+                            Real bv;
+                            if constexpr (sizeof(Real) == 4)
+                                bv = sm.gt[krow * S::CDM + 16 * n + c16];
+                            else
+                                bv = G[kk / 4][n][kk % 4];
+                            acc[m][n] = MfmaOps<Real>::mma(av, bv, acc[m][n]);
+                        }
                     }
-                }
+                });
             }
         }
         // the chunk's block into LDS (lane holds acc[ab = 16m + g + 4r][cd = 16n + c16]

@@ -40,8 +40,9 @@ struct BasisPlan {
     std::vector<PrimPair<float>> ppf;
     std::vector<int2> fnpairs;          // function pair id -> (i, j), i >= j
     // Hermite -> Cartesian matrices of every kept primitive pair (PairHdr::eoff):
-    // per primitive pair of a pair of class (la, lb), ncart(la) ncart(lb) rows
-    // (ab = a ncart(lb) + b) of nherm(la + lb) columns (packed tuv), entry
+    // per primitive pair of a pair of class (la, lb), nherm(la + lb) rows
+    // (packed tuv) of ncart(la) ncart(lb) columns (ab = a ncart(lb) + b, so
+    // the 16 lanes of an MFMA operand row read 16 consecutive values), entry
     // E^x_{a_x b_x, t} E^y_{a_y b_y, u} E^z_{a_z b_z, v} (zero where an
     // order exceeds a_k + b_k); FP64 and an FP32 copy.  etkd / etkf: the same
     // with column tuv scaled by (-1)^{t+u+v}, the sign the pair carries on the
@@ -79,19 +80,24 @@ inline void append_emat(int la, int lb, const PrimPair<double>& d, std::vector<d
                         std::vector<double>& etkd) {
     double ex[3][4][4][7];
     for (int x = 0; x < 3; ++x) herm1d_rt(la, lb, d.i2p, d.PA[x], d.PB[x], ex[x]);
-    const int L = la + lb;
+    const int L = la + lb, nab = ncart(la) * ncart(lb);
+    const size_t base = etd.size();
+    etd.resize(base + (size_t)nab * nherm(L));
+    etkd.resize(etd.size());
     for (int a = 0; a < ncart(la); ++a)
         for (int b = 0; b < ncart(lb); ++b) {
             const int ax = cart_x(la, a), ay = cart_y(la, a), az = cart_z(la, a);
             const int bx = cart_x(lb, b), by = cart_y(lb, b), bz = cart_z(lb, b);
+            const int ab = a * ncart(lb) + b;
+            int tuv = 0;
             for (int s = 0; s <= L; ++s)
                 for (int t = 0; t <= s; ++t)
-                    for (int u = 0; u <= s - t; ++u) {
+                    for (int u = 0; u <= s - t; ++u, ++tuv) {
                         const int v = s - t - u;
                         const bool nz = t <= ax + bx && u <= ay + by && v <= az + bz;
                         const double x = nz ? ex[0][ax][bx][t] * ex[1][ay][by][u] * ex[2][az][bz][v] : 0.0;
-                        etd.push_back(x);
-                        etkd.push_back((s & 1) ? -x : x);
+                        etd[base + (size_t)tuv * nab + ab] = x;
+                        etkd[base + (size_t)tuv * nab + ab] = (s & 1) ? -x : x;
                     }
         }
 }
