@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libhfx.so")
+# HFX_LIBRARY selects another build of the same library (the CPU sanitizer
+# builds of tests/sanitize); unset, the in-tree product build is loaded.
+LIB_PATH = os.environ.get("HFX_LIBRARY") or os.path.join(HERE, "_lib", "libhfx.so")
 
 HFX_MODE_STORED, HFX_MODE_DIRECT, HFX_MODE_AUTO = 0, 1, 2
 HFX_PREC_FP64, HFX_PREC_MIXED = 0, 1

@@ -12,7 +12,8 @@ import pytest
 from tests import oracle_py as orc
 from tests.conftest import GOLDEN, ROOT, golden
 
-EMU_PATH = os.path.join(ROOT, "tests", "emu", "libhfx_emu.so")
+# HFX_EMU_LIBRARY: a sanitizer build of the same library (tests/sanitize)
+EMU_PATH = os.environ.get("HFX_EMU_LIBRARY") or os.path.join(ROOT, "tests", "emu", "libhfx_emu.so")
 if not os.path.exists(EMU_PATH):
     pytest.skip("CPU emulation library not built (run __graft_entry__.build())", allow_module_level=True)
 emu = C.CDLL(EMU_PATH)
