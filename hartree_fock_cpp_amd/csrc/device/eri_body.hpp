@@ -273,8 +273,11 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
                 const int mu = q.fn1 + a, nu = q.fn2 + b;
                 if (q.d12 && mu < nu) continue;
                 const int e = (a * NB + b) * NCD + a * ND + b;
-                double v = (double)acc[e] * (fs[mu] * fs[nu] * fs[mu] * fs[nu]);
-                ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
+                const double v = (double)acc[e] * (fs[mu] * fs[nu] * fs[mu] * fs[nu]);
+                if (partial)
+                    atomic_add(&ep.out[pair_of(mu, nu)], v);
+                else
+                    ep.out[pair_of(mu, nu)] = v;
             }
         return;
     }
@@ -733,7 +736,10 @@ HFX_HD void famb_epilogue(SM& sm, const QuartetCtx& q, const EpiArgs& ep, int cd
             const int mu = q.fn1 + w / S::NB, nu = q.fn2 + w % S::NB;
             if (q.d12 && mu < nu) continue;
             const double v = (double)sm.acc[w * CDC + (cd - cd0)] * fs4(w, cd);
-            ep.out[pair_of(mu, nu)] = v > 0.0 ? sqrt(v) : 0.0;
+            if (nslice > 1)
+                atomic_add(&ep.out[pair_of(mu, nu)], v);
+            else
+                ep.out[pair_of(mu, nu)] = v;
         }
     } else if constexpr (JK) {
         // final masked values in place, then every J/K sum reads LDS only

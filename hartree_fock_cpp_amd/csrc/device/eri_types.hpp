@@ -37,7 +37,8 @@ struct PairHdr {
 };
 
 enum EpiMode : int {
-    EPI_DIAG = 0,     // Schwarz: bounds[pair(mu,nu)] = sqrt(max((mu nu|mu nu), 0))
+    EPI_DIAG = 0,     // Schwarz: out[pair(mu,nu)] (+)= (mu nu|mu nu) (primitive slices add);
+                      // the caller then takes bounds = sqrt(max(out, 0)) (launch_sqrt_clamp)
     EPI_UNIQUE = 1,   // write the 8-fold unique array at eri_index(mu,nu,la,si)
     EPI_JK = 2,       // contract with densities into J / K accumulators
 };

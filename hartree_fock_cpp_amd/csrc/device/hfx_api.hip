@@ -416,6 +416,23 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target
 
 // Plans the screened quartets of a basis and uploads them (cached per basis
 // for the FP64 unique build).
+// Unique-array plans whose total primitive work is at most this many primitive
+// quartets also slice their family-A quartets: such builds (the serving
+// stream's molecules) are set by their longest quartet, while on large ones
+// (benzene / cc-pVDZ: 1.1e8) family-A slices cost more than they balance.
+constexpr long long SMALL_PLAN_WORK = 1LL << 25;
+
+bool small_plan(const BasisPlan& bp, const QuartetLists& ql) {
+    long long s = 0;
+    for (int qc = 0; qc < NQCLASS; ++qc)
+        for (int p = 0; p < 2; ++p)
+            for (const uint2& e : ql.list[qc][p]) {
+                s += (long long)bp.hdr[e.x].nprim * bp.hdr[e.y].nprim;
+                if (s > SMALL_PLAN_WORK) return false;
+            }
+    return true;
+}
+
 int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets target, DevLists& dl) {
     dfree(dl.d);
     dl.valid = false;
@@ -423,6 +440,7 @@ int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets ta
     try {
         QuartetLists ql;
         plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql);
+        if (target.fam_a == 0 && small_plan(b->plan, ql)) target.fam_a = slice_target();
         if (int rc = upload_lists(b, ql, target, dl)) return rc;
     } catch (const std::bad_alloc&) {
         dfree(dl.d);
@@ -480,13 +498,21 @@ int compute_bounds_impl(hfx_basis* b) {
             return b->plan.hdr[x.x].nprim * b->plan.hdr[x.y].nprim > b->plan.hdr[y.x].nprim * b->plan.hdr[y.y].nprim;
         });
     }
+    // Quartets longer than a resident slot's share are cut into primitive
+    // slices (a 12-primitive s shell alone gives 20,736 primitive quartets):
+    // the epilogue leaves (mu nu|mu nu), slices adding into the zeroed
+    // output, and the square roots follow in one pass.
     DevLists dl;
-    if (int rc = upload_lists(b, ql, SliceTargets{0, 0}, dl)) return rc;
+    if (int rc = upload_lists(b, ql, jk_slices(), dl)) return rc;
     HIPCHK(launch_zero(b->d_bounds, P, b->stream));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_DIAG;
     ep.out = b->d_bounds;
     int rc = launch_plan(b, dl, ep, b->stream);
+    if (rc == 0) {
+        const hipError_t e = launch_sqrt_clamp(b->d_bounds, P, b->stream);
+        if (e != hipSuccess) rc = fail(HFX_ERR_DEVICE, std::string("schwarz bounds: ") + hipGetErrorString(e));
+    }
     if (rc == 0) {
         b->bounds.resize(P);
         hipError_t e = hipMemcpyAsync(b->bounds.data(), b->d_bounds, P * sizeof(double), hipMemcpyDeviceToHost, b->stream);

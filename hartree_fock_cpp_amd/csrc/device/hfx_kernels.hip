@@ -91,6 +91,18 @@ __global__ void k_zero(double* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
+__global__ void k_sqrt_clamp(double* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = p[i] > 0.0 ? sqrt(p[i]) : 0.0;
+}
+
+hipError_t launch_sqrt_clamp(double* p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    k_sqrt_clamp<<<(unsigned)(blocks < 1024 ? blocks : 1024), 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
 __global__ void k_axpy(double* y, const double* x, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += x[i];
 }

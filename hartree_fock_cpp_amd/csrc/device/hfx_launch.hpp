@@ -31,6 +31,7 @@ hipError_t launch_jk_rows(const double* eri, const int2* pairs, int K, int nspin
 hipError_t launch_symmetrize(double* M, int K, hipStream_t st);
 hipError_t launch_symmetrize_n(double* M0, double* M1, double* M2, int nmat, int K, hipStream_t st);   // nmat <= 3
 hipError_t launch_zero(double* p, size_t n, hipStream_t st);
+hipError_t launch_sqrt_clamp(double* p, size_t n, hipStream_t st);   // p = sqrt(max(p, 0))
 hipError_t launch_axpy(double* y, const double* x, size_t n, hipStream_t st);   // y += x
 hipError_t launch_add(double* z, const double* x, const double* y, size_t n, hipStream_t st);   // z = x + y
 // incremental J/K builds

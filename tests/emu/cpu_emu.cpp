@@ -5,6 +5,7 @@
 // class dispatch, Hermite/R recursions, contractions and all three epilogues
 // can be checked against the oracle without a GPU.  It is never loaded by the
 // product library, smoke() or bench.py.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -37,8 +38,9 @@ int g_force_slices = 0;
 
 std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q, int n, const EpiArgs& ep, int kb) {
     std::vector<uint2> out;
-    // the device slices J/K plans, and family B only in unique-array plans
-    if (ep.mode == EPI_DIAG || g_force_slices == 1 || (g_force_slices == 0 && ep.mode == EPI_UNIQUE && fam_a)) {
+    // the device slices J/K and Schwarz-diagonal plans, and family B only in
+    // (large) unique-array plans
+    if (g_force_slices == 1 || (g_force_slices == 0 && ep.mode == EPI_UNIQUE && fam_a)) {
         out.assign(q, q + n);
     } else if (g_force_slices < 0) {
         for (int k = 0; k < n; ++k) {
@@ -152,6 +154,7 @@ int make_ctx(int K, const double* xyz, const int* lmn, const int* poff, const do
     ep.mode = EPI_DIAG;
     ep.out = out.data();
     run_lists(c, ql, ep);
+    for (double& v : out) v = v > 0.0 ? std::sqrt(v) : 0.0;   // the epilogue leaves (mu nu|mu nu)
     c.bounds = out;
     shell_pair_bounds(c.plan, c.bounds.data(), c.Qsh);
     return 0;
