@@ -16,7 +16,7 @@
 
 namespace hfx {
 
-constexpr int FAM_A_BLOCK = 256;
+constexpr int FAM_A_BLOCK = 64;
 
 // G lanes per work unit; FAM_A_BLOCK / G units per workgroup.
 template <int LA, int LB, int LC, int LD, class Real, int G, bool JK>
