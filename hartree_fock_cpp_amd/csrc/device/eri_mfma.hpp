@@ -62,22 +62,27 @@ template <> struct MfmaOps<float> {
 };
 #endif
 
-template <int LA, int LB, int LC, int LD>
+// PGV: bra primitive pairs per group; 0 selects the class default PGD.  The
+// ket-component chunking (NT, CDM, NCHUNK) and the pass capacity NCUBE are
+// those of the default in every variant, so all variants share one LDS layout.
+template <int LA, int LB, int LC, int LD, int PGV = 0>
 struct FamMShape {
     using Bs = FamBShape<LA, LB, LC, LD>;
     static constexpr int NAB = Bs::NAB, NCD = Bs::NCD;
     static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
     static constexpr int NHAB = nherm(LAB), NHCD = nherm(LCD), NR = nherm(L);
-    // bra primitive pairs per group: stage-1 rows PG x NHAB within 64 where possible
+    // default: bra primitive pairs per group with stage-1 rows PGD x NHAB within 64 where possible
     static constexpr int PG0 = 64 / NHAB;
-    static constexpr int PG = PG0 < 1 ? 1 : PG0 > 4 ? 4 : PG0;
+    static constexpr int PGD = PG0 < 1 ? 1 : PG0 > 4 ? 4 : PG0;
+    static constexpr int PG = PGV > 0 ? PGV : PGD;
     static constexpr int MT1 = (PG * NHAB + 15) / 16;
     static constexpr int KT1 = (NHCD + 3) / 4;
     static constexpr int MT2 = (NAB + 15) / 16;
     static constexpr int KT2 = (PG * NHAB + 3) / 4;
     // ket component chunk: accumulators (MT1 + MT2) x NT tiles of 4 values per lane, at most 64
+    static constexpr int MT1D = (PGD * NHAB + 15) / 16;
     static constexpr int NTMAX = (NCD + 15) / 16;
-    static constexpr int NTFIT = 16 / (MT1 + MT2) < 1 ? 1 : 16 / (MT1 + MT2);
+    static constexpr int NTFIT = 16 / (MT1D + MT2) < 1 ? 1 : 16 / (MT1D + MT2);
     static constexpr int NT = NTMAX < NTFIT ? NTMAX : NTFIT;
     static constexpr int CDM = 16 * NT;
     static constexpr int NCHUNK = (NCD + CDM - 1) / CDM;
@@ -85,9 +90,10 @@ struct FamMShape {
     // (LDS per wavefront bounds the resident wavefronts per CU; measured:
     // staging the ket and bra matrices in LDS as well, one resident wavefront
     // per SIMD, ran (dp|pp) 1.8x slower than reading them from L2)
-    static constexpr int QC0 = 1024 / (PG * NR);
-    static constexpr int QC = QC0 < 1 ? 1 : QC0 > 64 ? 64 : QC0;
-    static constexpr int NCUBE = QC * PG;
+    static constexpr int QC0 = 1024 / (PGD * NR);
+    static constexpr int QCD = QC0 < 1 ? 1 : QC0 > 64 ? 64 : QC0;
+    static constexpr int NCUBE = QCD * PGD;
+    static constexpr int QC = NCUBE / PG;
     // stage-1 gather offsets held per lane in registers when few
     static constexpr bool OFFREG = MT1 * KT1 <= 12;
 };
@@ -147,22 +153,16 @@ __device__ __forceinline__ int famm_off(const SM& sm, int m, int kb, int lane) {
 // copies) in the precision of the class kernel.
 // pp / tab: the FP64 primitive pairs and Boys table (both precisions
 // evaluate the Boys function and prefactors in FP64).
-template <int LA, int LB, int LC, int LD, class Real, bool JK>
-__device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const PrimPair<double>* __restrict__ pp,
-                             const Real* __restrict__ etab, const Real* __restrict__ etabk,
-                             const double* __restrict__ tab, const EpiArgs& ep,
-                             FamMSmem<LA, LB, LC, LD, Real, JK>& sm, int lane) {
-    using S = FamMShape<LA, LB, LC, LD>;
+template <int LA, int LB, int LC, int LD, class Real, bool JK, int PGV>
+__device__ void family_m_body(const PairHdr& hb, const PairHdr& hk, const FamBRange& rg, int np, int nslice,
+                              const QuartetCtx& q, const PrimPair<double>* __restrict__ pp,
+                              const Real* __restrict__ etab, const Real* __restrict__ etabk,
+                              const double* __restrict__ tab, const EpiArgs& ep,
+                              FamMSmem<LA, LB, LC, LD, Real, JK>& sm, int lane) {
+    using S = FamMShape<LA, LB, LC, LD, PGV>;
     using V4 = typename MfmaOps<Real>::V4;
     auto sync = []() { __syncthreads(); };
-    const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
-    const int slice = unit_slice(e), nslice = unit_nslice(e);
-    const FamBRange rg = famb_range(slice, nslice, hb.nprim, hk.nprim, famb_kb(S::L));
-    if (rg.i0 >= hb.nprim || rg.j0 >= rg.j1 || dens_skip(ep, e, hb, hk)) return;   // uniform
-    const QuartetCtx q = make_ctx(hb, hk);
     const double fdeg = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
-    famb_stage_epi<LA, LB, LC, LD, Real, JK>(sm, q, ep, lane, 64);
-    const int np = (hb.nprim - rg.i0 + rg.istep - 1) / rg.istep;   // bra primitive pairs of this unit
     const int g = lane >> 4, c16 = lane & 15;                        // MFMA lane coordinates
     const Real* eb = etab + hb.eoff;
     const Real* ek = etabk + hk.eoff;
@@ -335,6 +335,32 @@ __device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const Pri
         famb_epilogue<LA, LB, LC, LD, S::CDM, JK, Real>(sm, q, ep, cd0, ncd, nslice, fdeg, lane, 64, sync);
         sync();
     }
+}
+
+// One work unit on one wavefront.  A unit with a single bra primitive pair
+// (common for d and f shells of one primitive) takes the PG = 1 body: with the
+// class's default group it would run its stage-1 tiles, stage-2 k-blocks, Boys
+// and R recursions on rows of absent bra pairs.
+template <int LA, int LB, int LC, int LD, class Real, bool JK>
+__device__ void family_m_one(uint2 e, const PairHdr* __restrict__ hdr, const PrimPair<double>* __restrict__ pp,
+                             const Real* __restrict__ etab, const Real* __restrict__ etabk,
+                             const double* __restrict__ tab, const EpiArgs& ep,
+                             FamMSmem<LA, LB, LC, LD, Real, JK>& sm, int lane) {
+    using S = FamMShape<LA, LB, LC, LD>;
+    const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
+    const int slice = unit_slice(e), nslice = unit_nslice(e);
+    const FamBRange rg = famb_range(slice, nslice, hb.nprim, hk.nprim, famb_kb(S::L));
+    if (rg.i0 >= hb.nprim || rg.j0 >= rg.j1 || dens_skip(ep, e, hb, hk)) return;   // uniform
+    const QuartetCtx q = make_ctx(hb, hk);
+    famb_stage_epi<LA, LB, LC, LD, Real, JK>(sm, q, ep, lane, 64);
+    const int np = (hb.nprim - rg.i0 + rg.istep - 1) / rg.istep;   // bra primitive pairs of this unit
+    if constexpr (S::PGD > 1) {
+        if (np == 1) {
+            family_m_body<LA, LB, LC, LD, Real, JK, 1>(hb, hk, rg, np, nslice, q, pp, etab, etabk, tab, ep, sm, lane);
+            return;
+        }
+    }
+    family_m_body<LA, LB, LC, LD, Real, JK, 0>(hb, hk, rg, np, nslice, q, pp, etab, etabk, tab, ep, sm, lane);
 }
 #endif
 
