@@ -414,8 +414,6 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target
     return 0;
 }
 
-// Plans the screened quartets of a basis and uploads them (cached per basis
-// for the FP64 unique build).
 // Unique-array plans whose total primitive work is at most this many primitive
 // quartets also slice their family-A quartets: such builds (the serving
 // stream's molecules) are set by their longest quartet, while on large ones
@@ -433,6 +431,8 @@ bool small_plan(const BasisPlan& bp, const QuartetLists& ql) {
     return true;
 }
 
+// Plans the screened quartets of a basis and uploads them (cached per basis
+// for the FP64 unique build).
 int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets target, DevLists& dl) {
     dfree(dl.d);
     dl.valid = false;
