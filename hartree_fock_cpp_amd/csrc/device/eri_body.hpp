@@ -60,12 +60,14 @@ __host__ __device__ constexpr bool use_family_a(int la, int lb, int lc, int ld) 
 
 // Family M (eri_mfma.hpp: both Hermite -> Cartesian contractions on the
 // matrix cores) runs the classes whose ket Hermite and Cartesian sizes (at
-// least 4 and 6) keep the MFMA tiles from being mostly padding: every such
-// class beyond family A's reach, and the family-A class (pp|pp), which it runs
-// 2x faster in the benzene / cc-pVDZ unique build (measured; (ds|ds) ran 1.5x
-// slower there and stays in family A).
+// least 4 and 3: a ket of p or more) keep the MFMA tiles from being mostly
+// padding: every such class beyond family A's reach, and the family-A class
+// (pp|pp), which it runs 2x faster in the benzene / cc-pVDZ unique build
+// (measured; (ds|ds) ran 1.5x slower there and stays in family A).  Moving the
+// (xx|ps) classes from family B (6 ket components required before) took the
+// benzene build 2.31 -> 2.19 ms and the (H2O)4 direct build 33.7 -> 31.8 ms.
 __host__ __device__ constexpr bool use_family_m(int la, int lb, int lc, int ld) {
-    if (nherm(lc + ld) < 4 || ncart(lc) * ncart(ld) < 6) return false;
+    if (nherm(lc + ld) < 4 || ncart(lc) * ncart(ld) < 3) return false;
     if (!use_family_a(la, lb, lc, ld)) return true;
     return la == 1 && lb == 1 && lc == 1 && ld == 1;
 }

@@ -132,7 +132,7 @@ def kernel_family(qc, fp32=False):
     nout = len(_cart(a)) * len(_cart(b)) * len(_cart(c)) * len(_cart(d))
     fam_a = L <= 4 and nout + math.comb(L + 3, 3) <= 128
     ncd = len(_cart(c)) * len(_cart(d))
-    fam_m = math.comb(c + d + 3, 3) >= 4 and ncd >= 6 and (not fam_a or (a, b, c, d) == (1, 1, 1, 1))
+    fam_m = math.comb(c + d + 3, 3) >= 4 and ncd >= 3 and (not fam_a or (a, b, c, d) == (1, 1, 1, 1))
     if fam_m:
         return "k_fam_m"
     return "k_fam_a" if fam_a else "k_fam_b"
