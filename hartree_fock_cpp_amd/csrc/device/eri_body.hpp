@@ -244,6 +244,255 @@ struct FamA {
     }
 };
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Value of lane (lane - SHIFT) of the lane's 16-lane row (DPP row_shr), 0 when
+// that lane lies in another row.
+template <int SHIFT>
+__device__ __forceinline__ double dpp_row_shr(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x110 + SHIFT, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + SHIFT, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int LANE>
+__device__ __forceinline__ double read_lane(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, LANE);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), LANE);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Segmented inclusive scan over the group leaders of one wavefront (lanes 0,
+// G, 2G, ... for G <= 16).  A run starts at every leader whose `head` flag is
+// set (lane 0 must set it) and ends before the next such leader; keys may
+// recur in later runs.  After scan(x) the last leader of each run (`tail`)
+// holds the sum of x over the run's leaders.  In-row steps use DPP row_shr by
+// G, 2G, 4G, 8G; the rows are then chained through lanes 15, 31, 47 (for G = 1;
+// 16 r - G in general) with readlane.  Every lane of the wavefront must be
+// active, and x must be 0 on lanes that are not leaders.
+template <int G>
+struct LeaderScan {
+    static constexpr int NIN = G == 1 ? 4 : G == 2 ? 3 : G == 4 ? 2 : G == 8 ? 1 : 0;
+    double m[NIN > 0 ? NIN : 1];   // 1.0 where the in-row source leader is in the run
+    double c[3];                   // 1.0 where the run reaches back to lane 16 r - G
+    bool tail;
+    __device__ __forceinline__ void init(bool head, int lane) {
+        const bool lead = lane % G == 0;
+        const unsigned long long H = __ballot(lead && head);
+        const int first = 63 - __clzll(H & (~0ull >> (63 - lane)));   // first leader of the lane's run
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) m[k] = (lane & 15) >= (G << k) && lane - (G << k) >= first ? 1.0 : 0.0;
+#pragma unroll
+        for (int r = 1; r <= 3; ++r) c[r - 1] = (lane >> 4) == r && first <= 16 * r - G ? 1.0 : 0.0;
+        tail = lead && (lane + G >= 64 || ((H >> (lane + G)) & 1ull) != 0);
+    }
+    __device__ __forceinline__ double scan(double x) const {
+        if constexpr (NIN > 0) x = fma(dpp_row_shr<G>(x), m[0], x);
+        if constexpr (NIN > 1) x = fma(dpp_row_shr<2 * G>(x), m[1], x);
+        if constexpr (NIN > 2) x = fma(dpp_row_shr<4 * G>(x), m[2], x);
+        if constexpr (NIN > 3) x = fma(dpp_row_shr<8 * G>(x), m[3], x);
+        x = fma(read_lane<16 - G>(x), c[0], x);
+        x = fma(read_lane<32 - G>(x), c[1], x);
+        x = fma(read_lane<48 - G>(x), c[2], x);
+        return x;
+    }
+};
+#endif
+
+// J/K epilogue of a register block acc[NOUT] (raw, unscaled) for the
+// kernel-order shell quartet q.  Every operand is loaded before the first
+// atomic (the compiler cannot move a load above an atomic that may alias it,
+// so interleaving them would serialise one memory latency per term), then the
+// block is scaled and masked once, the J and K sums formed in registers, and
+// the atomics issued back to back.
+//
+// WAVE = false: one lane, its own quartet (`lead` must be true).
+// WAVE = true (device, family A with G <= 16 lanes per unit): every lane of
+// the wavefront calls; `lead` marks the group leaders whose unit has work.
+// The planner orders a J/K list by (bra pair, ket first shell, ket pair)
+// inside each primitive-work class, so neighbouring units of a wavefront
+// share J_ab (key: bra pair) and K_ac, K_bc (key: bra pair and ket shell c)
+// targets: those sums are added along runs of equal key (LeaderScan) and only
+// the last unit of a run issues their atomics; J_cd, K_ad and K_bd stay one
+// atomic per unit.  Units past the list end pass active = false.
+template <int NA, int NB, int NC, int ND, class Real, int G, bool WAVE>
+HFX_HD __forceinline__ void epilogue_jk(const QuartetCtx& q, const EpiArgs& ep, const Real* acc, bool lead,
+                                        bool active, int bra, int cshell, int lane) {
+    constexpr int NCD = NC * ND, NAB = NA * NB;
+    const int K = ep.K;
+    const double* fs = ep.fscale;
+    const double f = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
+    double v[NAB * NCD];
+    double jab[NAB], jcd[NCD];
+    double dac[NA * NC], dad[NA * ND], dbc[NB * NC], dbd[NB * ND];
+#pragma unroll
+    for (int x = 0; x < NAB; ++x) jab[x] = 0.0;
+#pragma unroll
+    for (int y = 0; y < NCD; ++y) jcd[y] = 0.0;
+    auto load_k = [&](const double* D) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) dac[a * NC + c] = D[(q.fn1 + a) * K + q.fn3 + c];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) dad[a * ND + d] = D[(q.fn1 + a) * K + q.fn4 + d];
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) dbc[b * NC + c] = D[(q.fn2 + b) * K + q.fn3 + c];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) dbd[b * ND + d] = D[(q.fn2 + b) * K + q.fn4 + d];
+        }
+    };
+    if (lead) {
+        double fsa[NA], fsb[NB], fsc[NC], fsd[ND];
+        double bab[NAB], bcd[NCD], djab[NAB], djcd[NCD];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) fsa[a] = fs[q.fn1 + a];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) fsb[b] = fs[q.fn2 + b];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) fsc[c] = fs[q.fn3 + c];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) fsd[d] = fs[q.fn4 + d];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                bab[a * NB + b] = ep.bounds[pair_of(q.fn1 + a, q.fn2 + b)];
+                djab[a * NB + b] = ep.Dj[(q.fn1 + a) * K + q.fn2 + b];
+            }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                bcd[c * ND + d] = ep.bounds[pair_of(q.fn3 + c, q.fn4 + d)];
+                djcd[c * ND + d] = ep.Dj[(q.fn3 + c) * K + q.fn4 + d];
+            }
+        load_k(ep.Da);
+#pragma unroll
+        for (int x = 0; x < NAB; ++x)
+#pragma unroll
+            for (int y = 0; y < NCD; ++y) {
+                const int a = x / NB, b = x % NB, c = y / ND, d = y % ND;
+                v[x * NCD + y] = bab[x] * bcd[y] >= ep.threshold
+                                     ? (double)acc[x * NCD + y] * (fsa[a] * fsb[b] * fsc[c] * fsd[d]) : 0.0;
+            }
+#pragma unroll
+        for (int x = 0; x < NAB; ++x)
+#pragma unroll
+            for (int y = 0; y < NCD; ++y) {
+                jab[x] += v[x * NCD + y] * djcd[y];
+                jcd[y] += v[x * NCD + y] * djab[x];
+            }
+#pragma unroll
+        for (int x = 0; x < NAB; ++x) jab[x] *= 2.0 * f;
+#pragma unroll
+        for (int y = 0; y < NCD; ++y) jcd[y] *= 2.0 * f;
+    } else {
+#pragma unroll
+        for (int e = 0; e < NAB * NCD; ++e) v[e] = 0.0;
+    }
+    bool tail_ab = lead, tail_c = lead;
+#if defined(__HIP_DEVICE_COMPILE__)
+    LeaderScan<G> sab, sc;
+    if constexpr (WAVE) {
+        const int pbra = __shfl_up(bra, G, 64), pc = __shfl_up(cshell, G, 64);
+        const bool head_ab = lane < G || pbra != bra;
+        sab.init(head_ab, lane);
+        sc.init(head_ab || pc != cshell, lane);
+        tail_ab = active && sab.tail;
+        tail_c = active && sc.tail;
+#pragma unroll
+        for (int x = 0; x < NAB; ++x) jab[x] = sab.scan(jab[x]);
+    }
+#endif
+    // K sums of one spin: Kacc_ac += f sum_bd v D_bd, Kacc_bc += f sum_ad v D_ad,
+    // Kacc_ad += f sum_bc v D_bc, Kacc_bd += f sum_ac v D_ac
+    auto k_terms = [&](double* Kacc) {
+        double kac[NA * NC], kbc[NB * NC], kad[NA * ND], kbd[NB * ND];
+#pragma unroll
+        for (int x = 0; x < NA * NC; ++x) kac[x] = 0.0;
+#pragma unroll
+        for (int x = 0; x < NB * NC; ++x) kbc[x] = 0.0;
+#pragma unroll
+        for (int x = 0; x < NA * ND; ++x) kad[x] = 0.0;
+#pragma unroll
+        for (int x = 0; x < NB * ND; ++x) kbd[x] = 0.0;
+        if (lead) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+#pragma unroll
+                        for (int d = 0; d < ND; ++d) {
+                            const double x = v[(a * NB + b) * NCD + c * ND + d];
+                            kac[a * NC + c] += x * dbd[b * ND + d];
+                            kbc[b * NC + c] += x * dad[a * ND + d];
+                            kad[a * ND + d] += x * dbc[b * NC + c];
+                            kbd[b * ND + d] += x * dac[a * NC + c];
+                        }
+#pragma unroll
+            for (int x = 0; x < NA * NC; ++x) kac[x] *= f;
+#pragma unroll
+            for (int x = 0; x < NB * NC; ++x) kbc[x] *= f;
+#pragma unroll
+            for (int x = 0; x < NA * ND; ++x) kad[x] *= f;
+#pragma unroll
+            for (int x = 0; x < NB * ND; ++x) kbd[x] *= f;
+        }
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (WAVE) {
+#pragma unroll
+            for (int x = 0; x < NA * NC; ++x) kac[x] = sc.scan(kac[x]);
+#pragma unroll
+            for (int x = 0; x < NB * NC; ++x) kbc[x] = sc.scan(kbc[x]);
+        }
+#endif
+        if (tail_c) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) atomic_add(&Kacc[(q.fn1 + a) * K + q.fn3 + c], kac[a * NC + c]);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) atomic_add(&Kacc[(q.fn2 + b) * K + q.fn3 + c], kbc[b * NC + c]);
+        }
+        if (lead) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) atomic_add(&Kacc[(q.fn1 + a) * K + q.fn4 + d], kad[a * ND + d]);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) atomic_add(&Kacc[(q.fn2 + b) * K + q.fn4 + d], kbd[b * ND + d]);
+        }
+    };
+    k_terms(ep.Ka);
+    if (tail_ab) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) atomic_add(&ep.J[(q.fn1 + a) * K + q.fn2 + b], jab[a * NB + b]);
+    }
+    if (lead) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) atomic_add(&ep.J[(q.fn3 + c) * K + q.fn4 + d], jcd[c * ND + d]);
+    }
+    if (ep.nspin == 2) {
+        if (lead) load_k(ep.Db);
+        k_terms(ep.Kb);
+    }
+}
+
 // Family-A epilogue over a register block acc[NOUT] (raw, unscaled).
 template <int NA, int NB, int NC, int ND, class Real, bool JK>
 HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep, const Real* acc, bool partial) {
@@ -284,135 +533,7 @@ HFX_HD __forceinline__ void epilogue_regs(const QuartetCtx& q, const EpiArgs& ep
         return;
     }
     } else {
-    // EPI_JK.  Every operand is loaded before the first atomic (the
-    // compiler cannot move a load above an atomic that may alias it, so
-    // interleaving them would serialise one memory latency per term), then
-    // the block is scaled and masked once, the J and K sums formed in
-    // registers, and the atomics issued back to back.
-    const int K = ep.K;
-    const double f = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
-    constexpr int NAB = NA * NB;
-    double fsa[NA], fsb[NB], fsc[NC], fsd[ND];
-    double bab[NAB], bcd[NCD], djab[NAB], djcd[NCD];
-    double dac[NA * NC], dad[NA * ND], dbc[NB * NC], dbd[NB * ND];
-#pragma unroll
-    for (int a = 0; a < NA; ++a) fsa[a] = fs[q.fn1 + a];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) fsb[b] = fs[q.fn2 + b];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) fsc[c] = fs[q.fn3 + c];
-#pragma unroll
-    for (int d = 0; d < ND; ++d) fsd[d] = fs[q.fn4 + d];
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            bab[a * NB + b] = ep.bounds[pair_of(q.fn1 + a, q.fn2 + b)];
-            djab[a * NB + b] = ep.Dj[(q.fn1 + a) * K + q.fn2 + b];
-        }
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            bcd[c * ND + d] = ep.bounds[pair_of(q.fn3 + c, q.fn4 + d)];
-            djcd[c * ND + d] = ep.Dj[(q.fn3 + c) * K + q.fn4 + d];
-        }
-    auto load_k = [&](const double* D) {
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) dac[a * NC + c] = D[(q.fn1 + a) * K + q.fn3 + c];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) dad[a * ND + d] = D[(q.fn1 + a) * K + q.fn4 + d];
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) dbc[b * NC + c] = D[(q.fn2 + b) * K + q.fn3 + c];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) dbd[b * ND + d] = D[(q.fn2 + b) * K + q.fn4 + d];
-        }
-    };
-    load_k(ep.Da);
-    double v[NA * NB * NCD];
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    const int e = (a * NB + b) * NCD + c * ND + d;
-                    v[e] = bab[a * NB + b] * bcd[c * ND + d] >= ep.threshold
-                               ? (double)acc[e] * (fsa[a] * fsb[b] * fsc[c] * fsd[d]) : 0.0;
-                }
-    double jab[NAB], jcd[NCD];
-#pragma unroll
-    for (int x = 0; x < NAB; ++x) jab[x] = 0.0;
-#pragma unroll
-    for (int y = 0; y < NCD; ++y) jcd[y] = 0.0;
-#pragma unroll
-    for (int x = 0; x < NAB; ++x)
-#pragma unroll
-        for (int y = 0; y < NCD; ++y) {
-            jab[x] += v[x * NCD + y] * djcd[y];
-            jcd[y] += v[x * NCD + y] * djab[x];
-        }
-    // K sums of one spin: Kacc_ac += f sum_bd v D_bd, Kacc_bc += f sum_ad v D_ad,
-    // Kacc_ad += f sum_bc v D_bc, Kacc_bd += f sum_ac v D_ac
-    auto k_terms = [&](double* Kacc) {
-        double kac[NA * NC], kbc[NB * NC], kad[NA * ND], kbd[NB * ND];
-#pragma unroll
-        for (int x = 0; x < NA * NC; ++x) kac[x] = 0.0;
-#pragma unroll
-        for (int x = 0; x < NB * NC; ++x) kbc[x] = 0.0;
-#pragma unroll
-        for (int x = 0; x < NA * ND; ++x) kad[x] = 0.0;
-#pragma unroll
-        for (int x = 0; x < NB * ND; ++x) kbd[x] = 0.0;
-#pragma unroll
-        for (int a = 0; a < NA; ++a)
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int c = 0; c < NC; ++c)
-#pragma unroll
-                    for (int d = 0; d < ND; ++d) {
-                        const double x = v[(a * NB + b) * NCD + c * ND + d];
-                        kac[a * NC + c] += x * dbd[b * ND + d];
-                        kbc[b * NC + c] += x * dad[a * ND + d];
-                        kad[a * ND + d] += x * dbc[b * NC + c];
-                        kbd[b * ND + d] += x * dac[a * NC + c];
-                    }
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) atomic_add(&Kacc[(q.fn1 + a) * K + q.fn3 + c], f * kac[a * NC + c]);
-#pragma unroll
-            for (int d = 0; d < ND; ++d) atomic_add(&Kacc[(q.fn1 + a) * K + q.fn4 + d], f * kad[a * ND + d]);
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) atomic_add(&Kacc[(q.fn2 + b) * K + q.fn3 + c], f * kbc[b * NC + c]);
-#pragma unroll
-            for (int d = 0; d < ND; ++d) atomic_add(&Kacc[(q.fn2 + b) * K + q.fn4 + d], f * kbd[b * ND + d]);
-        }
-    };
-    k_terms(ep.Ka);
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) atomic_add(&ep.J[(q.fn1 + a) * K + q.fn2 + b], 2.0 * f * jab[a * NB + b]);
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int d = 0; d < ND; ++d) atomic_add(&ep.J[(q.fn3 + c) * K + q.fn4 + d], 2.0 * f * jcd[c * ND + d]);
-    if (ep.nspin == 2) {
-        load_k(ep.Db);
-        k_terms(ep.Kb);
-    }
+    epilogue_jk<NA, NB, NC, ND, Real, 1, false>(q, ep, acc, true, true, 0, 0, 0);
     }
 }
 
@@ -459,6 +580,14 @@ HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const 
         for (int off = G / 2; off > 0; off >>= 1)
 #pragma unroll
             for (int i = 0; i < F::NOUT; ++i) acc[i] += __shfl_xor(acc[i], off, G);
+    }
+#endif
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (JK && G < 64) {
+        epilogue_jk<F::NA, F::NB, F::NC, F::ND, Real, G, true>(make_ctx(hb, hk), ep, acc, work && sub == 0, active,
+                                                               active ? (int)unit_bra(e) : -1, active ? hk.s1 : -1,
+                                                               (int)__lane_id());
+        return;
     }
 #endif
     if (work && sub == 0)

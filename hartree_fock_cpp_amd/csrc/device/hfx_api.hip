@@ -432,14 +432,16 @@ bool small_plan(const BasisPlan& bp, const QuartetLists& ql) {
 }
 
 // Plans the screened quartets of a basis and uploads them (cached per basis
-// for the FP64 unique build).
-int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets target, DevLists& dl) {
+// for the FP64 unique build).  jk_order: the J/K ordering of plan_quartets
+// (direct builds).
+int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets target, DevLists& dl,
+              bool jk_order = false) {
     dfree(dl.d);
     dl.valid = false;
     // host work in the C ABI's call: failures come back as status codes
     try {
         QuartetLists ql;
-        plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql);
+        plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql, 0, jk_order);
         if (target.fam_a == 0 && small_plan(b->plan, ql)) target.fam_a = slice_target();
         if (int rc = upload_lists(b, ql, target, dl)) return rc;
     } catch (const std::bad_alloc&) {
@@ -891,7 +893,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
         }
     } else {
         prc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split,
-                        mode == HFX_MODE_STORED ? unique_slices() : jk_slices(), f->plan);
+                        mode == HFX_MODE_STORED ? unique_slices() : jk_slices(), f->plan, mode != HFX_MODE_STORED);
     }
     if (prc) {
         hfx_fock_destroy(f);
@@ -1128,7 +1130,7 @@ int replan(hfx_fock* f, long long t) {
         f->graph = nullptr;
     }
     if (int rc = make_plan(f->b, f->opts.threshold, f->opts.precision == HFX_PREC_MIXED, f->opts.mixed_split,
-                           SliceTargets{t, t}, f->plan))
+                           SliceTargets{t, t}, f->plan, f->mode != HFX_MODE_STORED))
         return rc;
     f->plan_target = t;
     return 0;

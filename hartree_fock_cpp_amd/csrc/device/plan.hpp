@@ -325,7 +325,11 @@ inline void parallel_tasks(int ntask, int nthreads, F&& fn) {
 // function quartet inside it can pass the function-level mask otherwise).
 // In mixed precision, quartets with Q_x Q_y < split go to the FP32 list.
 // Within a class, quartets are ordered by primitive work w (descending) so
-// that lanes of a wavefront run loops of similar length.
+// that lanes of a wavefront run loops of similar length.  With jk_order (J/K
+// plans), quartets of equal w are further ordered by bra pair, then by the
+// ket pair's first shell (kernel order), then by ket pair: neighbouring units
+// of a family-A wavefront then share the J_ab and K_ac / K_bc targets, which
+// the J/K epilogue adds up along the wavefront before one atomic per run.
 //
 // Two passes over the pair triangle (count, then place) and a stable counting
 // sort on w per list: linear in the number of quartets (13.6 M for
@@ -335,7 +339,7 @@ inline void parallel_tasks(int ntask, int nthreads, F&& fn) {
 // out in the same order as from one thread.  threads > 0 overrides the
 // automatic thread count (tests compare thread counts).
 inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, double thr, bool mixed, double split,
-                          QuartetLists& ql, int threads = 0) {
+                          QuartetLists& ql, int threads = 0, bool jk_order = false) {
     constexpr int NL = 2 * NQCLASS;
     const int np = bp.npair;
     std::vector<int> nprim(np), pcls(np);
@@ -391,12 +395,36 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
     int maxw = 0;
     for (int x = 0; x < np; ++x) maxw = std::max(maxw, nprim[x]);
     maxw = maxw * maxw;
+    // jk_order: rank of each pair in (first shell, pair id) order
+    std::vector<int> kr;
+    if (jk_order) {
+        std::vector<int> start((size_t)bp.nshell + 1, 0);
+        for (int x = 0; x < np; ++x) ++start[(size_t)bp.hdr[x].s1 + 1];
+        for (int sh = 0; sh < bp.nshell; ++sh) start[(size_t)sh + 1] += start[sh];
+        kr.resize(np);
+        for (int x = 0; x < np; ++x) kr[x] = start[bp.hdr[x].s1]++;
+    }
+    // stable counting sort of n entries from src to dst by key(e) in [0, nkey)
+    auto counting_sort = [](const uint2* src, uint2* dst, size_t n, size_t nkey, auto key) {
+        std::vector<size_t> hist(nkey + 1, 0);
+        for (size_t i = 0; i < n; ++i) ++hist[key(src[i]) + 1];
+        for (size_t k = 0; k < nkey; ++k) hist[k + 1] += hist[k];
+        for (size_t i = 0; i < n; ++i) dst[hist[key(src[i])]++] = src[i];
+    };
     parallel_tasks(NL, T, [&](int l) {
         const int c = l / 2, p = l % 2;
         const uint2* v = raw.get() + lstart[l];
         const size_t n = lstart[l + 1] - lstart[l];
         std::vector<uint2>& out = ql.list[c][p];
         out.resize(n);
+        std::vector<uint2> byket, bybra;
+        if (jk_order && n > 1) {   // least significant key first: ket rank, then bra
+            byket.resize(n);
+            bybra.resize(n);
+            counting_sort(v, byket.data(), n, (size_t)np, [&](const uint2& e) { return (size_t)kr[e.y]; });
+            counting_sort(byket.data(), bybra.data(), n, (size_t)np, [](const uint2& e) { return (size_t)e.x; });
+            v = bybra.data();
+        }
         // stable counting sort, w descending: slot of w starts after all larger w
         std::vector<size_t> hist((size_t)maxw + 2, 0);
         for (size_t i = 0; i < n; ++i) ++hist[(size_t)(maxw - nprim[v[i].x] * nprim[v[i].y])];

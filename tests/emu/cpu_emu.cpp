@@ -10,7 +10,9 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
+#include <algorithm>
 #include <vector>
 
 #include "../../hartree_fock_cpp_amd/csrc/device/plan.hpp"
@@ -200,7 +202,7 @@ int emu_fock_direct(EMU_BASIS_ARGS, double threshold, int mixed, double split, c
     EmuCtx c;
     if (make_ctx(K, xyz, lmn, poff, alpha, coeff, norm, c)) return -1;
     QuartetLists ql;
-    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, ql);
+    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, ql, 0, true);   // the device's J/K order
     const size_t K2 = (size_t)K * K;
     std::vector<double> Dj(Pa, Pa + K2);
     if (Pb)
@@ -266,6 +268,39 @@ int emu_plan_threads_agree(EMU_BASIS_ARGS, double threshold, int mixed, double s
             for (size_t i = 0; same && i < a.size(); ++i) same = a[i].x == b[i].x && a[i].y == b[i].y;
             for (int g = 0; same && g < NGROUP; ++g) same = one.bucket[q][p][g] == many.bucket[q][p][g];
             bad += same ? 0 : 1;
+        }
+    return bad;
+}
+
+// Plans the screened quartets in the default order and in the J/K order (on
+// `threads` threads) and checks every J/K list: the same quartets and
+// lane-group buckets as the default list, primitive work w descending, and
+// within equal w ascending (bra pair, ket first shell, ket pair) with no
+// repeats.  Returns the number of lists that fail (0: all pass) and the total
+// quartet count in *nq.
+int emu_plan_jk_order_check(EMU_BASIS_ARGS, double threshold, int mixed, double split, int threads, long long* nq) {
+    EmuCtx c;
+    if (make_ctx(K, xyz, lmn, poff, alpha, coeff, norm, c)) return -1;
+    QuartetLists def, jk;
+    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, def, threads);
+    plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, jk, threads, true);
+    const auto& H = c.plan.hdr;
+    auto w = [&](const uint2& e) { return (long long)H[e.x].nprim * H[e.y].nprim; };
+    auto key = [&](const uint2& e) { return std::make_tuple(-w(e), e.x, H[e.y].s1, e.y); };
+    int bad = 0;
+    *nq = (long long)jk.total();
+    for (int q = 0; q < NQCLASS; ++q)
+        for (int p = 0; p < 2; ++p) {
+            std::vector<uint2> a = def.list[q][p];
+            const std::vector<uint2>& b = jk.list[q][p];
+            bool ok = a.size() == b.size();
+            for (int g = 0; ok && g < NGROUP; ++g) ok = def.bucket[q][p][g] == jk.bucket[q][p][g];
+            for (size_t i = 1; ok && i < b.size(); ++i) ok = key(b[i - 1]) < key(b[i]);
+            if (ok) {
+                std::sort(a.begin(), a.end(), [&](const uint2& x, const uint2& y) { return key(x) < key(y); });
+                for (size_t i = 0; ok && i < a.size(); ++i) ok = a[i].x == b[i].x && a[i].y == b[i].y;
+            }
+            bad += ok ? 0 : 1;
         }
     return bad;
 }

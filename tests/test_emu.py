@@ -192,6 +192,24 @@ def test_plan_threads_agree(threads, mixed):
     assert nq.value > 1_000_000
 
 
+@pytest.mark.parametrize("threads,mixed", [(1, 0), (16, 1)])
+def test_plan_jk_order(threads, mixed):
+    """The J/K plan order: every class list holds the same quartets and
+    lane-group buckets as the default plan, ordered by primitive work
+    (descending), then bra pair, ket first shell and ket pair (ascending,
+    no repeats) -- the order the J/K epilogue's run merging relies on:
+    benzene / cc-pVDZ at 1e-8, FP64 and the mixed split."""
+    from hartree_fock_cpp_amd import workloads as W
+    from hartree_fock_cpp_amd.basis import basis_construction
+    d = basis_construction(W.benzene(), "cc-pVDZ")
+    K, keep, a = _bargs(d)
+    nq = C.c_longlong(0)
+    bad = emu.emu_plan_jk_order_check(*a, C.c_double(1e-8), C.c_int(mixed), C.c_double(1e-5), C.c_int(threads),
+                                      C.byref(nq))
+    assert bad == 0, (bad, emu.emu_last_error())
+    assert nq.value > 1_000_000
+
+
 @pytest.mark.parametrize("threads", [1, 4, 16])
 def test_parallel_tasks_propagates_exceptions(threads):
     """The planner's thread helper runs every task once when none throws, and
