@@ -98,6 +98,10 @@ template <> struct MathOps<float> {
 template <int M, class Real>
 HFX_HD __forceinline__ void boys_fm(Real T, const Real* __restrict__ tab, Real F[M + 1]) {
     constexpr int NT = sizeof(Real) == 8 ? BOYS_NTAYLOR : BOYS_NTAYLOR_F32;
+    // e^{-T} for either recursion, evaluated once before the branch: lanes of
+    // one wavefront often take both branches, which then share one exp
+    Real e = Real(0);
+    if constexpr (M > 0) e = MathOps<Real>::exp_(-T);
     if (T < Real(BOYS_TMAX)) {
         int i = (int)(T * Real(BOYS_INV_H) + Real(0.5));
         Real d = Real(i) * Real(BOYS_H) - T;   // -(T - T_i)
@@ -107,7 +111,6 @@ HFX_HD __forceinline__ void boys_fm(Real T, const Real* __restrict__ tab, Real F
         for (int k = NT - 2; k >= 0; --k) f = row[k] + f * d * Real(1.0 / (k + 1));
         F[M] = f;
         if (M > 0) {
-            Real e = MathOps<Real>::exp_(-T);
             Real T2 = T + T;
 #pragma unroll
             for (int m = M - 1; m >= 0; --m) F[m] = (T2 * F[m + 1] + e) * Real(1.0 / (2 * m + 1));
@@ -116,7 +119,6 @@ HFX_HD __forceinline__ void boys_fm(Real T, const Real* __restrict__ tab, Real F
         Real rT = MathOps<Real>::rcp(T);
         F[0] = Real(0.886226925452758013649083741671) * MathOps<Real>::sqrt_(rT);  // sqrt(pi)/2 / sqrt(T)
         if (M > 0) {
-            Real e = MathOps<Real>::exp_(-T);
             Real h = Real(0.5) * rT;
 #pragma unroll
             for (int m = 0; m < M; ++m) F[m + 1] = (Real(2 * m + 1) * F[m] - e) * h;
