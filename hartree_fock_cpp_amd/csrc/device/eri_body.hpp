@@ -174,28 +174,57 @@ struct FamA {
         });
     }
 
-    // Primitive loop of one shell quartet over the flattened (bra, ket)
-    // primitive-pair index f = i * nk + j, taking f = sub, sub + G, ... so
-    // that a group of G lanes shares the quartet; acc must be zeroed and holds
-    // this lane's partial sum.
+    // Primitive loops of one shell quartet on a group of G lanes.  The slice's
+    // bra rows are the bra primitive pairs slice, slice + nslice, ...  acc must
+    // be zeroed and holds this lane's partial sum.
+    //  * L <= 1 ((ss|ss), (ps|ss)): the lanes form a gi x gj grid (gi gj = G);
+    //    lane sub takes rows sub / gj, + gi, ... and, for each, ket pairs
+    //    sub % gj, + gj, ..., so a bra pair's data and E tables are loaded once
+    //    per row instead of once per ket step (these kernels are bound by
+    //    vector-load address processing).  gi is the power of two with the
+    //    fewest steps per lane, ceil(rows / gi) ceil(nk / gj) (ties: the larger
+    //    gi, fewer bra loads).
+    //  * otherwise the flattened (row, ket) index f = sub, sub + G, ... (no idle
+    //    lanes; measured faster for the classes with larger blocks).
     static HFX_HD __forceinline__ void quartet(const PairHdr& hb, const PairHdr& hk, const PrimPair<Real>* pp,
-                                               const Real* tab, Real* acc, int sub, int G) {
+                                               const Real* tab, Real* acc, int sub, int G, int slice, int nslice) {
         const int nk = hk.nprim;
-        const int tot = hb.nprim * nk;
-        int i = nk > 0 ? sub / nk : 0;
-        int j = sub - i * nk;
-        int last_i = -1;
+        const int rows = (hb.nprim - slice + nslice - 1) / nslice;
+        int gi = 1, r0, rstep, j0, jstep;
+        if constexpr (L <= 1) {
+            int best = 0x7fffffff;
+            for (int g = 1; g <= G; g <<= 1) {
+                const int gj = G / g;
+                const int c = ((rows + g - 1) / g) * ((nk + gj - 1) / gj);
+                if (c <= best) {
+                    best = c;
+                    gi = g;
+                }
+            }
+            r0 = sub / (G / gi);
+            rstep = gi;
+            j0 = sub % (G / gi);
+            jstep = G / gi;
+        } else {
+            r0 = nk > 0 ? sub / nk : 0;   // flattened: row and ket of f = sub
+            rstep = 0;
+            j0 = sub - r0 * nk;
+            jstep = G;
+        }
+        const int tot = rows * nk;
+        int r = r0, j = j0, f = sub;
+        int last_r = -1;
         PrimPair<Real> b;
         Herm1D<LA, LB, Real> ex, ey, ez;
-        for (int f = sub; f < tot; f += G) {
-            if (i != last_i) {
-                b = pp[hb.prim_off + i];
+        while (L <= 1 ? r < rows : f < tot) {
+            if (r != last_r) {
+                b = pp[hb.prim_off + slice + r * nslice];
                 ex.build(b.i2p, b.PA[0], b.PB[0]);
                 ey.build(b.i2p, b.PA[1], b.PB[1]);
                 ez.build(b.i2p, b.PA[2], b.PB[2]);
-                last_i = i;
+                last_r = r;
             }
-            {
+            if (L > 1 || j < nk) {
                 const PrimPair<Real> k = pp[hk.prim_off + j];
                 Herm1D<LC, LD, Real> fx, fy, fz;
                 fx.build(k.i2p, k.PA[0], k.PB[0]);
@@ -234,11 +263,18 @@ struct FamA {
                 hermite_R<L, Real>(a, X, Y, Z, R);
                 contract(acc, ex, ey, ez, fx, fy, fz, R);
             }
-            j += G;
-            if (j >= nk) {   // G may exceed nk when primitive slices widen the stride
-                const int di = j / nk;
-                i += di;
-                j -= di * nk;
+            // next step: the next ket of the row (grid) or f + G (flattened)
+            f += G;
+            j += jstep;
+            if (j >= nk) {
+                if constexpr (L <= 1) {
+                    r += rstep;
+                    j = j0;
+                } else {   // G may exceed nk when rows are short
+                    const int dr = j / nk;
+                    r += dr;
+                    j -= dr * nk;
+                }
             }
         }
     }
@@ -560,9 +596,9 @@ HFX_HD __forceinline__ QuartetCtx make_ctx(const PairHdr& hb, const PairHdr& hk)
 // dividing the wavefront).  `sub` is the lane's index inside its group; every
 // lane of the group must call this (inactive groups pass active = false) so
 // the cross-lane reduction sees the whole group.  Lane 0 of the group runs the
-// epilogue.  A unit that is slice s of S covers primitive quartets
-// s*G + sub + k*G*S of the flattened bra x ket loop, and its epilogue adds a
-// partial block (J/K and the unique array are linear in it).
+// epilogue.  A unit that is slice s of S covers the bra primitive pairs s,
+// s + S, ... against all ket pairs, and its epilogue adds a partial block
+// (J/K and the unique array are linear in it).
 template <int LA, int LB, int LC, int LD, class Real, int G, bool JK>
 HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const PairHdr* hdr, const PrimPair<Real>* pp,
                                            const Real* tab, const EpiArgs& ep) {
@@ -572,8 +608,8 @@ HFX_HD __forceinline__ void family_a_group(uint2 e, bool active, int sub, const 
     Real acc[F::NOUT];
 #pragma unroll
     for (int i = 0; i < F::NOUT; ++i) acc[i] = Real(0);
-    const bool work = active && slice * G < hb.nprim * hk.nprim && !dens_skip(ep, e, hb, hk);
-    if (work) F::quartet(hb, hk, pp, tab, acc, slice * G + sub, G * nslice);
+    const bool work = active && slice < hb.nprim && !dens_skip(ep, e, hb, hk);
+    if (work) F::quartet(hb, hk, pp, tab, acc, sub, G, slice, nslice);
 #if defined(__HIP_DEVICE_COMPILE__)
     if constexpr (G > 1) {
 #pragma unroll
@@ -612,11 +648,11 @@ inline void family_a_group_emu(uint2 e, int G, const PairHdr* hdr, const PrimPai
     using F = FamA<LA, LB, LC, LD, Real>;
     const PairHdr hb = hdr[unit_bra(e)], hk = hdr[unit_ket(e)];
     const int slice = unit_slice(e), nslice = unit_nslice(e);
-    if (slice * G >= hb.nprim * hk.nprim || dens_skip(ep, e, hb, hk)) return;
+    if (slice >= hb.nprim || dens_skip(ep, e, hb, hk)) return;
     Real tot[F::NOUT] = {};
     for (int sub = 0; sub < G; ++sub) {
         Real acc[F::NOUT] = {};
-        F::quartet(hb, hk, pp, tab, acc, slice * G + sub, G * nslice);
+        F::quartet(hb, hk, pp, tab, acc, sub, G, slice, nslice);
         for (int i = 0; i < F::NOUT; ++i) tot[i] += acc[i];
     }
     if (ep.mode == EPI_JK)

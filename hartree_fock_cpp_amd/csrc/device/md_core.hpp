@@ -61,20 +61,24 @@ __host__ __device__ constexpr int hidx(int t, int u, int v) {
 
 // ------------------------------------------------------------------ Boys
 // F_m(T) = int_0^1 x^{2m} exp(-T x^2) dx.
-// Table: F_n(T_i) at T_i = i*h, h = 1/20, i = 0..BOYS_NGRID-1, n = 0..BOYS_NCOL-1.
+// Table: F_n(T_i) at T_i = i*h, h = 1/80, i = 0..BOYS_NGRID-1, n = 0..BOYS_NCOL-1.
 // For T < BOYS_TMAX: Taylor expansion of F_mmax about the nearest grid point
-// (|dT| <= h/2, BOYS_NTAYLOR terms => relative error < 4e-18 in FP64), then the
-// stable downward recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
+// (|dT| <= h/2 = 1/160; BOYS_NTAYLOR = 6 terms leave (h/2)^6 / 6! < 1e-16 of
+// F_{m+6} <= F_m in FP64, 4 terms 7e-11 in FP32), then the stable downward
+// recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).  The fine grid keeps a
+// lookup to six table values (three 16-byte loads per lane; eight terms on a
+// 1/20 grid took four): the low-L family-A kernels are bound by the texture
+// address unit's load processing, not by arithmetic.
 // For T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2 (erfc(sqrt(40)) < 1e-19) and the
 // upward recursion, which is stable for T > m.
 constexpr int BOYS_MMAX = 12;        // (ff|ff) needs L = 12
-constexpr int BOYS_NTAYLOR = 8;
-constexpr int BOYS_NCOL = BOYS_MMAX + BOYS_NTAYLOR + 1;   // 21
-constexpr double BOYS_H = 0.05;
-constexpr double BOYS_INV_H = 20.0;
+constexpr int BOYS_NTAYLOR = 6;
+constexpr int BOYS_NCOL = BOYS_MMAX + BOYS_NTAYLOR + 1;   // 19
+constexpr double BOYS_H = 0.0125;
+constexpr double BOYS_INV_H = 80.0;
 constexpr double BOYS_TMAX = 40.0;
-constexpr int BOYS_NGRID = 801;      // T_i = 0 .. 40
-constexpr int BOYS_NTAYLOR_F32 = 5;
+constexpr int BOYS_NGRID = 3201;     // T_i = 0 .. 40
+constexpr int BOYS_NTAYLOR_F32 = 4;
 
 template <class Real> struct MathOps;
 template <> struct MathOps<double> {

@@ -448,8 +448,7 @@ inline void plan_quartets(const BasisPlan& bp, const std::vector<double>& Q, dou
 // resident on the device at once (`target` wavefronts: one unit per wavefront
 // in family B, 64 / G units per wavefront in family A) the launch needs about
 // total work / R per slot, so a quartet is cut into nslice = ceil(w / W)
-// primitive slices -- of the bra primitive loop (family B) or of the
-// flattened primitive loop (family A) -- with
+// slices of its bra primitive loop (at most one per bra primitive pair) with
 //   W = max(total work / R, w_min),
 // w_min = one bra primitive (family B) or four primitive quartets per lane
 // (family A).  Launches that fill the device keep their quartets whole
@@ -496,7 +495,7 @@ inline void expand_units(const BasisPlan& bp, bool fam_a, int G, const uint2* q,
         const long long w = (long long)nb * nk;
         if (fam_a) {
             const long long Wq = std::max<long long>(W, 4LL * G);
-            const long long cap = std::max<long long>(1, w / (4LL * G));
+            const long long cap = std::max<long long>(1, std::min<long long>(w / (4LL * G), nb));
             emit(q[k], std::max<long long>(1, std::min<long long>(std::min<long long>((w + Wq - 1) / Wq, cap), SLICE_MAX)));
             continue;
         }
@@ -523,21 +522,27 @@ inline void plan_diag(const BasisPlan& bp, std::vector<uint2> (&lists)[NQCLASS])
     }
 }
 
-// F_n(T_i) on the grid T_i = i h, n = 0..BOYS_NCOL-1, from the positive series
-// F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) in long double.
+// F_n(T_i) on the grid T_i = i h, n = 0..BOYS_NCOL-1, in long double: the top
+// column from the positive series
+//   F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)),
+// the others by the (stable) downward recursion F_n = (2T F_{n+1} + e^{-T}) / (2n+1).
 inline void boys_table(std::vector<double>& tab) {
     tab.assign((size_t)BOYS_NGRID * BOYS_NCOL, 0.0);
     for (int i = 0; i < BOYS_NGRID; ++i) {
         const long double T = (long double)i * (long double)BOYS_H;
         const long double e = expl(-T);
-        for (int n = 0; n < BOYS_NCOL; ++n) {
-            long double term = 1.0L / (2.0L * n + 1.0L), sum = term;
-            for (int k = 1; k < 4000; ++k) {
-                term *= 2.0L * T / (2.0L * n + 2.0L * k + 1.0L);
-                sum += term;
-                if (term < 1e-24L * sum) break;
-            }
-            tab[(size_t)i * BOYS_NCOL + n] = (double)(e * sum);
+        const int n = BOYS_NCOL - 1;
+        long double term = 1.0L / (2.0L * n + 1.0L), sum = term;
+        for (int k = 1; k < 4000; ++k) {
+            term *= 2.0L * T / (2.0L * n + 2.0L * k + 1.0L);
+            sum += term;
+            if (term < 1e-24L * sum) break;
+        }
+        long double f = e * sum;
+        tab[(size_t)i * BOYS_NCOL + n] = (double)f;
+        for (int m = n - 1; m >= 0; --m) {
+            f = (2.0L * T * f + e) / (2.0L * m + 1.0L);
+            tab[(size_t)i * BOYS_NCOL + m] = (double)f;
         }
     }
 }

@@ -49,7 +49,7 @@ std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q,
             const int nb = c.plan.hdr[q[k].x].nprim, nk = c.plan.hdr[q[k].y].nprim;
             int S;
             if (fam_a) {
-                S = std::min(-g_force_slices, std::max(1, nb * nk / G));
+                S = std::min(-g_force_slices, std::max(1, std::min(nb * nk / G, nb)));
             } else {
                 int nkc = std::min(-g_force_slices, (nk + kb - 1) / kb);
                 while (nkc > 1 && nb * nkc > SLICE_MAX) --nkc;
@@ -59,8 +59,8 @@ std::vector<uint2> emu_units(const EmuCtx& c, bool fam_a, int G, const uint2* q,
         }
     } else if (g_force_slices > 1) {
         for (int k = 0; k < n; ++k) {
-            const int cap = fam_a ? std::max(1, c.plan.hdr[q[k].x].nprim * c.plan.hdr[q[k].y].nprim / G)
-                                  : c.plan.hdr[q[k].x].nprim;
+            const int nb = c.plan.hdr[q[k].x].nprim;
+            const int cap = fam_a ? std::max(1, std::min(nb * c.plan.hdr[q[k].y].nprim / G, nb)) : nb;
             const int S = std::min(g_force_slices, cap);
             for (int sl = 0; sl < S; ++sl) out.push_back(make_uint2(q[k].x | ((unsigned)sl << 24), q[k].y | ((unsigned)S << 24)));
         }
