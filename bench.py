@@ -278,6 +278,9 @@ def direct_build(name, atoms, basis_set, steps, thr=1e-12, split=1e-5):
              "frac_fp64_peak": fl.sum() / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
              "shell_quartets": int(nq.sum()), "fp32_shell_quartets": int(nq[1::2].sum()),
              "serialized_class_ms": float(cls.sum()),
+             "top_classes_ms": {"%s %s fp%s" % (W.kernel_family(i // 2, i % 2 == 1), W.qclass_name(i // 2),
+                                                 "32" if i % 2 else "64"): round(float(cls[i]), 4)
+                                for i in np.argsort(-cls)[:12] if cls[i] > 0},
              "dominant": {"kernel": "%s %s fp%s" % (W.kernel_family(dom // 2, dom % 2 == 1), W.qclass_name(dom // 2),
                                                     "32" if dom % 2 else "64"),
                           "launch_ms": float(cls[dom]), "model_gflop": fl[dom] / 1e9,
