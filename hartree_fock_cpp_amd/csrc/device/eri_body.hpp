@@ -180,97 +180,124 @@ struct FamA {
     //  * L <= 1 ((ss|ss), (ps|ss)): the lanes form a gi x gj grid (gi gj = G);
     //    lane sub takes rows sub / gj, + gi, ... and, for each, ket pairs
     //    sub % gj, + gj, ..., so a bra pair's data and E tables are loaded once
-    //    per row instead of once per ket step (these kernels are bound by
-    //    vector-load address processing).  gi is the power of two with the
-    //    fewest steps per lane, ceil(rows / gi) ceil(nk / gj) (ties: the larger
-    //    gi, fewer bra loads).
+    //    per row (these kernels are bound by vector-load address processing);
+    //    (ss|ss) takes two rows per ket load.  gi is the power of two with the
+    //    fewest steps per lane (ties: the larger gi, fewer bra loads).
     //  * otherwise the flattened (row, ket) index f = sub, sub + G, ... (no idle
     //    lanes; measured faster for the classes with larger blocks).
+    // One primitive quartet (bra pair b, ket pair k with their E tables, the
+    // ket's odd Hermite orders negated) added into acc.
+    static HFX_HD __forceinline__ void prim(const PrimPair<Real>& b, const Herm1D<LA, LB, Real>& ex,
+                                            const Herm1D<LA, LB, Real>& ey, const Herm1D<LA, LB, Real>& ez,
+                                            const PrimPair<Real>& k, const Herm1D<LC, LD, Real>& fx,
+                                            const Herm1D<LC, LD, Real>& fy, const Herm1D<LC, LD, Real>& fz,
+                                            const Real* tab, Real* acc) {
+        Real s = b.p + k.p;
+        Real inv = Real(1) / s;
+        Real rho = b.p * k.p * inv;
+        Real X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
+        Real T = rho * (X * X + Y * Y + Z * Z);
+        Real F[L + 1];
+        boys_fm<L, Real>(T, tab, F);
+        Real pref = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
+        Real a[L + 1];
+        Real m2r = Real(-2) * rho;
+        Real fct = pref;
+#pragma unroll
+        for (int n = 0; n <= L; ++n) {
+            a[n] = fct * F[n];
+            fct *= m2r;
+        }
+        Real R[nherm(L)];
+        hermite_R<L, Real>(a, X, Y, Z, R);
+        contract(acc, ex, ey, ez, fx, fy, fz, R);
+    }
+
+    static HFX_HD __forceinline__ void ket_tables(const PrimPair<Real>& k, Herm1D<LC, LD, Real>& fx,
+                                                  Herm1D<LC, LD, Real>& fy, Herm1D<LC, LD, Real>& fz) {
+        fx.build(k.i2p, k.PA[0], k.PB[0]);
+        fy.build(k.i2p, k.PA[1], k.PB[1]);
+        fz.build(k.i2p, k.PA[2], k.PB[2]);
+        // (-1)^{tau+nu+phi}: negate odd Hermite orders of the ket tables
+        static_for<0, LC + 1>([&](auto ci) {
+            static_for<0, LD + 1>([&](auto di) {
+                static_for<0, LCD + 1>([&](auto ti) {
+                    constexpr int c = decltype(ci)::value, d = decltype(di)::value, t = decltype(ti)::value;
+                    if constexpr (t % 2 == 1) {
+                        fx.e[c][d][t] = -fx.e[c][d][t];
+                        fy.e[c][d][t] = -fy.e[c][d][t];
+                        fz.e[c][d][t] = -fz.e[c][d][t];
+                    }
+                });
+            });
+        });
+    }
+
     static HFX_HD __forceinline__ void quartet(const PairHdr& hb, const PairHdr& hk, const PrimPair<Real>* pp,
                                                const Real* tab, Real* acc, int sub, int G, int slice, int nslice) {
         const int nk = hk.nprim;
         const int rows = (hb.nprim - slice + nslice - 1) / nslice;
-        int gi = 1, r0, rstep, j0, jstep;
         if constexpr (L <= 1) {
-            int best = 0x7fffffff;
+            // lane grid gi x gj; for (ss|ss) a lane step takes RB = 2 of the
+            // lane's rows against one ket pair, so each ket load serves two
+            // primitive quartets (for (ps|ss) the second row's registers cost
+            // more occupancy than the loads save: RB = 1, measured):
+            // steps = ceil(ceil(rows / gi) / RB) ceil(nk / gj)
+            constexpr int RB = L == 0 ? 2 : 1;
+            int gi = 1, best = 0x7fffffff;
             for (int g = 1; g <= G; g <<= 1) {
                 const int gj = G / g;
-                const int c = ((rows + g - 1) / g) * ((nk + gj - 1) / gj);
+                const int c = (((rows + g - 1) / g + RB - 1) / RB) * ((nk + gj - 1) / gj);
                 if (c <= best) {
                     best = c;
                     gi = g;
                 }
             }
-            r0 = sub / (G / gi);
-            rstep = gi;
-            j0 = sub % (G / gi);
-            jstep = G / gi;
-        } else {
-            r0 = nk > 0 ? sub / nk : 0;   // flattened: row and ket of f = sub
-            rstep = 0;
-            j0 = sub - r0 * nk;
-            jstep = G;
-        }
-        const int tot = rows * nk;
-        int r = r0, j = j0, f = sub;
-        int last_r = -1;
-        PrimPair<Real> b;
-        Herm1D<LA, LB, Real> ex, ey, ez;
-        while (L <= 1 ? r < rows : f < tot) {
-            if (r != last_r) {
-                b = pp[hb.prim_off + slice + r * nslice];
-                ex.build(b.i2p, b.PA[0], b.PB[0]);
-                ey.build(b.i2p, b.PA[1], b.PB[1]);
-                ez.build(b.i2p, b.PA[2], b.PB[2]);
-                last_r = r;
+            const int gj = G / gi;
+            for (int r = sub / gj; r < rows; r += RB * gi) {
+                const bool two = RB == 2 && r + gi < rows;
+                const PrimPair<Real> b0 = pp[hb.prim_off + slice + r * nslice];
+                PrimPair<Real> b1 = b0;
+                if constexpr (RB == 2) b1 = pp[hb.prim_off + slice + (two ? r + gi : r) * nslice];
+                Herm1D<LA, LB, Real> ex0, ey0, ez0, ex1, ey1, ez1;
+                ex0.build(b0.i2p, b0.PA[0], b0.PB[0]);
+                ey0.build(b0.i2p, b0.PA[1], b0.PB[1]);
+                ez0.build(b0.i2p, b0.PA[2], b0.PB[2]);
+                if constexpr (RB == 2) {
+                    ex1.build(b1.i2p, b1.PA[0], b1.PB[0]);
+                    ey1.build(b1.i2p, b1.PA[1], b1.PB[1]);
+                    ez1.build(b1.i2p, b1.PA[2], b1.PB[2]);
+                }
+                for (int j = sub % gj; j < nk; j += gj) {
+                    const PrimPair<Real> k = pp[hk.prim_off + j];
+                    Herm1D<LC, LD, Real> fx, fy, fz;
+                    ket_tables(k, fx, fy, fz);
+                    prim(b0, ex0, ey0, ez0, k, fx, fy, fz, tab, acc);
+                    if constexpr (RB == 2)
+                        if (two) prim(b1, ex1, ey1, ez1, k, fx, fy, fz, tab, acc);
+                }
             }
-            if (L > 1 || j < nk) {
+        } else {
+            // flattened (row, ket) index f = sub, sub + G, ...
+            const int tot = rows * nk;
+            int r = nk > 0 ? sub / nk : 0, j = sub - r * nk;
+            int last_r = -1;
+            PrimPair<Real> b;
+            Herm1D<LA, LB, Real> ex, ey, ez;
+            for (int f = sub; f < tot; f += G) {
+                if (r != last_r) {
+                    b = pp[hb.prim_off + slice + r * nslice];
+                    ex.build(b.i2p, b.PA[0], b.PB[0]);
+                    ey.build(b.i2p, b.PA[1], b.PB[1]);
+                    ez.build(b.i2p, b.PA[2], b.PB[2]);
+                    last_r = r;
+                }
                 const PrimPair<Real> k = pp[hk.prim_off + j];
                 Herm1D<LC, LD, Real> fx, fy, fz;
-                fx.build(k.i2p, k.PA[0], k.PB[0]);
-                fy.build(k.i2p, k.PA[1], k.PB[1]);
-                fz.build(k.i2p, k.PA[2], k.PB[2]);
-                // (-1)^{tau+nu+phi}: negate odd Hermite orders of the ket tables
-                static_for<0, LC + 1>([&](auto ci) {
-                    static_for<0, LD + 1>([&](auto di) {
-                        static_for<0, LCD + 1>([&](auto ti) {
-                            constexpr int c = decltype(ci)::value, d = decltype(di)::value, t = decltype(ti)::value;
-                            if constexpr (t % 2 == 1) {
-                                fx.e[c][d][t] = -fx.e[c][d][t];
-                                fy.e[c][d][t] = -fy.e[c][d][t];
-                                fz.e[c][d][t] = -fz.e[c][d][t];
-                            }
-                        });
-                    });
-                });
-                Real s = b.p + k.p;
-                Real inv = Real(1) / s;
-                Real rho = b.p * k.p * inv;
-                Real X = b.P[0] - k.P[0], Y = b.P[1] - k.P[1], Z = b.P[2] - k.P[2];
-                Real T = rho * (X * X + Y * Y + Z * Z);
-                Real F[L + 1];
-                boys_fm<L, Real>(T, tab, F);
-                Real pref = Real(TWO_PI_POW_2_5) * b.w * k.w * MathOps<Real>::sqrt_(inv);
-                Real a[L + 1];
-                Real m2r = Real(-2) * rho;
-                Real fct = pref;
-#pragma unroll
-                for (int n = 0; n <= L; ++n) {
-                    a[n] = fct * F[n];
-                    fct *= m2r;
-                }
-                Real R[nherm(L)];
-                hermite_R<L, Real>(a, X, Y, Z, R);
-                contract(acc, ex, ey, ez, fx, fy, fz, R);
-            }
-            // next step: the next ket of the row (grid) or f + G (flattened)
-            f += G;
-            j += jstep;
-            if (j >= nk) {
-                if constexpr (L <= 1) {
-                    r += rstep;
-                    j = j0;
-                } else {   // G may exceed nk when rows are short
+                ket_tables(k, fx, fy, fz);
+                prim(b, ex, ey, ez, k, fx, fy, fz, tab, acc);
+                j += G;
+                if (j >= nk) {   // G may exceed nk when rows are short
                     const int dr = j / nk;
                     r += dr;
                     j -= dr * nk;
