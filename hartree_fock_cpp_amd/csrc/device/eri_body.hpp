@@ -900,9 +900,13 @@ HFX_HD __forceinline__ void build_e_lanes(Real* dst, const PrimPair<Real>& pp, b
 // sm.acc[ab * CDC + cd - cd0]; sm.epi holds the staged scales, bounds and
 // densities (famb_stage_epi).  Shared by the VALU and the MFMA kernels.
 template <int LA, int LB, int LC, int LD, int CDC, bool JK, class Real, class SM, class Sync>
-HFX_HD void famb_epilogue(SM& sm, const QuartetCtx& q, const EpiArgs& ep, int cd0, int ncd, int nslice, double fdeg,
-                          int lane, int nlanes, Sync sync) {
+HFX_HD void famb_epilogue(SM& sm, const QuartetCtx& q, const EpiArgs& ep, int cd0_in, int ncd_in, int nslice,
+                          double fdeg, int lane, int nlanes, Sync sync) {
     using S = FamBShape<LA, LB, LC, LD>;
+    // one chunk holds every ket component pair: compile-time bounds let the
+    // compiler unroll the sums below and drop their range checks
+    constexpr bool ONE = S::NCD <= CDC;
+    const int cd0 = ONE ? 0 : cd0_in, ncd = ONE ? S::NCD : ncd_in;
     const int K = ep.K;
     const double* E = sm.epi;
     auto fs4 = [&](int ab, int cd) -> double {
