@@ -517,9 +517,10 @@ def main():
                    "K": basis.K, "Q_total": Q, "Q_screened": q_screened, "threshold": thr,
                    "shell_quartets": int(nq.sum()), "primitive_quartets": int(nprimq.sum()),
                    "model_gflop_per_build": flops_cls.sum() / 1e9},
-        "roofline": {"bound": "mfma" if fam_dom == "k_fam_m" else "valu",
-                     "pipe": "fp64 %s (FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak)"
-                             % ("MFMA" if fam_dom == "k_fam_m" else "VALU"),
+        "roofline": {"bound": "mfma",
+                     "pipe": "FP64 compute roofline; the kernel's FP64 work issues on the %s "
+                             "(FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak)"
+                             % ("matrix cores" if fam_dom == "k_fam_m" else "VALU"),
                      "kernel": "%s %s fp%s" % (fam_dom, W.qclass_name(dom // 2),
                                                "32" if dom % 2 else "64"),
                      "achieved": dom_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
