@@ -79,9 +79,11 @@ __host__ __device__ constexpr bool runs_family_a(int la, int lb, int lc, int ld)
 }
 
 // ------------------------------------------------------------------ epilogue
-// A shell quartet block v[a][b][c][d] (already scaled to final values) for the
-// kernel-order shells (s1 s2 | s3 s4) is consumed in one of three ways.  The
-// accessor `val(a,b,c,d)` returns the block value in FP64.
+// A shell quartet block v[a][b][c][d] for the kernel-order shells
+// (s1 s2 | s3 s4) is consumed in one of three ways: the Schwarz diagonal and
+// the unique array take the final values (raw block times the four function
+// scales); EPI_JK takes the raw block, the scales being folded into its
+// densities and outputs (F = diag(fscale): J = F J'(F Dj F) F, the same for K).
 //
 // Degeneracy for EPI_JK: with f = 1/|stabiliser| of the shell quartet under
 // the 8-fold permutation group, summing all 8 images over the full block is
@@ -364,11 +366,11 @@ struct LeaderScan {
 #endif
 
 // J/K epilogue of a register block acc[NOUT] (raw, unscaled) for the
-// kernel-order shell quartet q.  Every operand is loaded before the first
-// atomic (the compiler cannot move a load above an atomic that may alias it,
-// so interleaving them would serialise one memory latency per term), then the
-// block is scaled and masked once, the J and K sums formed in registers, and
-// the atomics issued back to back.
+// kernel-order shell quartet q, against densities that carry the function
+// scales.  Every operand is loaded before the first atomic (the compiler
+// cannot move a load above an atomic that may alias it, so interleaving them
+// would serialise one memory latency per term), then the block is masked once,
+// the J and K sums formed in registers, and the atomics issued back to back.
 //
 // WAVE = false: one lane, its own quartet (`lead` must be true).
 // WAVE = true (device, family A with G <= 16 lanes per unit): every lane of
@@ -384,7 +386,6 @@ HFX_HD __forceinline__ void epilogue_jk(const QuartetCtx& q, const EpiArgs& ep, 
                                         bool active, int bra, int cshell, int lane) {
     constexpr int NCD = NC * ND, NAB = NA * NB;
     const int K = ep.K;
-    const double* fs = ep.fscale;
     const double f = (q.d12 ? 0.5 : 1.0) * (q.d34 ? 0.5 : 1.0) * (q.same ? 0.5 : 1.0);
     double v[NAB * NCD];
     double jab[NAB], jcd[NCD];
@@ -410,16 +411,7 @@ HFX_HD __forceinline__ void epilogue_jk(const QuartetCtx& q, const EpiArgs& ep, 
         }
     };
     if (lead) {
-        double fsa[NA], fsb[NB], fsc[NC], fsd[ND];
         double bab[NAB], bcd[NCD], djab[NAB], djcd[NCD];
-#pragma unroll
-        for (int a = 0; a < NA; ++a) fsa[a] = fs[q.fn1 + a];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) fsb[b] = fs[q.fn2 + b];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) fsc[c] = fs[q.fn3 + c];
-#pragma unroll
-        for (int d = 0; d < ND; ++d) fsd[d] = fs[q.fn4 + d];
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -435,14 +427,13 @@ HFX_HD __forceinline__ void epilogue_jk(const QuartetCtx& q, const EpiArgs& ep, 
                 djcd[c * ND + d] = ep.Dj[(q.fn3 + c) * K + q.fn4 + d];
             }
         load_k(ep.Da);
+        // the raw block: the function scales sit in the densities and outputs
+        // (enqueue_direct scales them), only the Schwarz mask applies here
 #pragma unroll
         for (int x = 0; x < NAB; ++x)
 #pragma unroll
-            for (int y = 0; y < NCD; ++y) {
-                const int a = x / NB, b = x % NB, c = y / ND, d = y % ND;
-                v[x * NCD + y] = bab[x] * bcd[y] >= ep.threshold
-                                     ? (double)acc[x * NCD + y] * (fsa[a] * fsb[b] * fsc[c] * fsd[d]) : 0.0;
-            }
+            for (int y = 0; y < NCD; ++y)
+                v[x * NCD + y] = bab[x] * bcd[y] >= ep.threshold ? (double)acc[x * NCD + y] : 0.0;
 #pragma unroll
         for (int x = 0; x < NAB; ++x)
 #pragma unroll
@@ -940,11 +931,12 @@ HFX_HD void famb_epilogue(SM& sm, const QuartetCtx& q, const EpiArgs& ep, int cd
                 ep.out[pair_of(mu, nu)] = v;
         }
     } else if constexpr (JK) {
-        // final masked values in place, then every J/K sum reads LDS only
+        // masked raw values in place (the function scales sit in the densities
+        // and outputs of a direct build), then every J/K sum reads LDS only
         for (int w = lane; w < S::NAB * ncd; w += nlanes) {
             const int ab = w / ncd, cdl = w % ncd, cd = cd0 + cdl;
             Real& a = sm.acc[ab * CDC + cdl];
-            a = E[S::O_BAB + ab] * E[S::O_BCD + cd] >= ep.threshold ? (Real)((double)a * fs4(ab, cd)) : Real(0);
+            if (E[S::O_BAB + ab] * E[S::O_BCD + cd] < ep.threshold) a = Real(0);
         }
         sync();
         auto v = [&](int ab, int cdl) -> double { return (double)sm.acc[ab * CDC + cdl]; };

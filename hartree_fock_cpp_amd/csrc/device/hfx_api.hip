@@ -183,6 +183,9 @@ struct hfx_fock {
     // incremental direct builds: previous densities, density differences,
     // accumulated (symmetrised) J / K, shell-block maxima of |dP|
     double* d_inc = nullptr;
+    // direct builds: the densities scaled by the function component scales
+    // (Dj, Da, Db, each diag(fs) D diag(fs))
+    double* d_scl = nullptr;
     // pinned staging of the host-matrix entry point (Pa, Pb in; J, Ka, Kb out)
     double* h_stage = nullptr;
     double* h_stage_dev = nullptr;
@@ -900,7 +903,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
         return prc;
     }
     for (int qc = 0; qc < NQCLASS; ++qc) f->nq += f->plan.nquart[qc][0] + f->plan.nquart[qc][1];
-    const size_t direct_bytes = f->nq * sizeof(uint2) + mats;
+    const size_t direct_bytes = f->nq * sizeof(uint2) + mats + 3 * K2 * sizeof(double);
     f->mode = mode;
     f->bytes = mode == HFX_MODE_STORED ? stored_bytes : direct_bytes;
     if (opts->hbm_budget && f->bytes > opts->hbm_budget) {
@@ -913,6 +916,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
         rc = fail(HFX_ERR_DEVICE, "hipEventCreate");
     if (!rc && opts->incremental && mode == HFX_MODE_DIRECT)
         rc = dalloc(&f->d_inc, 8 * K2 + (size_t)b->plan.nshell * b->plan.nshell);
+    if (!rc && mode == HFX_MODE_DIRECT) rc = dalloc(&f->d_scl, 3 * K2);
     f->plan_target = slice_target();
     if (!rc && mode == HFX_MODE_DIRECT && autotune_on() && f->plan_target > 0) {
         f->tune_target[0] = f->plan_target;   // the plan just made
@@ -956,6 +960,7 @@ void hfx_fock_destroy(hfx_fock* f) {
         if (e) (void)hipEventDestroy(e);
     dfree(f->d_eri);
     dfree(f->d_inc);
+    dfree(f->d_scl);
     if (f->h_stage) hfx_detail::host_free(f->h_stage);
     dfree(f->plan.d);
     dfree(f->d_Dj);   // one allocation for all six matrices
@@ -1018,7 +1023,11 @@ bool fock_graphs_on() {
 // kernel launches, no memset / memcpy nodes.
 // Direct J/K from densities (Dj for J; Da, Db for K) into zeroed, then
 // symmetrised accumulators; dpmax != nullptr screens units by the density
-// difference bound.
+// difference bound.  The J/K epilogues work in the basis of the unscaled
+// Cartesian components: with F = diag(fscale) the block is v = F4 raw, so
+// J = F J'(F Dj F) F and K = F K'(F D F) F where J', K' contract the raw
+// block.  The densities are scaled once here and the outputs while being
+// symmetrised, so no epilogue loads a function scale.
 int enqueue_direct(hfx_fock* f, const double* Dj, const double* d_Pa, const double* d_Pb, double* d_J, double* d_Ka,
                    double* d_Kb, const double* dpmax, hipStream_t st) {
     hfx_basis* b = f->b;
@@ -1026,13 +1035,17 @@ int enqueue_direct(hfx_fock* f, const double* Dj, const double* d_Pa, const doub
     const size_t K2 = (size_t)K * K;
     const int nspin = d_Pb ? 2 : 1;
     HIPCHK(zero_jk(d_J, d_Ka, d_Kb, nspin, K2, st));
+    double* sDj = f->d_scl;
+    double* sDa = f->d_scl + K2;
+    double* sDb = f->d_scl + 2 * K2;
+    HIPCHK(launch_scale_n(sDj, sDa, sDb, Dj, d_Pa, d_Pb, 1 + nspin, b->d_fscale, K, st));
     EpiArgs ep = base_epi(b);
     ep.mode = EPI_JK;
     ep.threshold = f->opts.threshold;
     ep.nspin = nspin;
-    ep.Dj = Dj;
-    ep.Da = d_Pa;
-    ep.Db = d_Pb;
+    ep.Dj = sDj;
+    ep.Da = sDa;
+    ep.Db = nspin == 2 ? sDb : nullptr;
     ep.J = d_J;
     ep.Ka = d_Ka;
     ep.Kb = d_Kb;
@@ -1043,7 +1056,7 @@ int enqueue_direct(hfx_fock* f, const double* Dj, const double* d_Pa, const doub
         ep.dthr = incremental_tau(f);
     }
     if (int rc = launch_plan(b, f->plan, ep, st)) return rc;
-    HIPCHK(launch_symmetrize_n(d_J, d_Ka, d_Kb, 1 + nspin, K, st));
+    HIPCHK(launch_symmetrize_n(d_J, d_Ka, d_Kb, 1 + nspin, K, st, b->d_fscale));
     return 0;
 }
 

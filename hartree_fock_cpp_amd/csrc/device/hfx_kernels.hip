@@ -58,8 +58,9 @@ __global__ void k_symmetrize(double* M, int K) {
     M[j * K + i] = s;
 }
 
-// The same for up to three matrices in one launch (blockIdx.y picks one).
-__global__ void k_symmetrize3(double* M0, double* M1, double* M2, int K) {
+// The same for up to three matrices in one launch (blockIdx.y picks one);
+// fs != nullptr also scales entry (i, j) by fs[i] fs[j].
+__global__ void k_symmetrize3(double* M0, double* M1, double* M2, int K, const double* fs) {
     double* M = blockIdx.y == 0 ? M0 : blockIdx.y == 1 ? M1 : M2;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t np = (int64_t)K * (K + 1) / 2;
@@ -68,15 +69,39 @@ __global__ void k_symmetrize3(double* M0, double* M1, double* M2, int K) {
     while ((int64_t)i * (i + 1) / 2 > p) --i;
     while ((int64_t)(i + 1) * (i + 2) / 2 <= p) ++i;
     const int j = (int)(p - (int64_t)i * (i + 1) / 2);
-    const double s = M[i * K + j] + M[j * K + i];
+    double s = M[i * K + j] + M[j * K + i];
+    if (fs) s *= fs[i] * fs[j];
     M[i * K + j] = s;
     M[j * K + i] = s;
 }
 
-hipError_t launch_symmetrize_n(double* M0, double* M1, double* M2, int nmat, int K, hipStream_t st) {
+hipError_t launch_symmetrize_n(double* M0, double* M1, double* M2, int nmat, int K, hipStream_t st,
+                               const double* fs) {
     const int64_t np = (int64_t)K * (K + 1) / 2;
     const int block = 256;
-    k_symmetrize3<<<dim3((unsigned)((np + block - 1) / block), (unsigned)nmat), block, 0, st>>>(M0, M1, M2, K);
+    k_symmetrize3<<<dim3((unsigned)((np + block - 1) / block), (unsigned)nmat), block, 0, st>>>(M0, M1, M2, K, fs);
+    return hipGetLastError();
+}
+
+// out_m[i K + j] = fs[i] fs[j] in_m[i K + j] for up to three K x K matrices
+// (blockIdx.y picks one).
+__global__ void k_scale3(double* O0, double* O1, double* O2, const double* I0, const double* I1, const double* I2,
+                         const double* fs, int K) {
+    double* O = blockIdx.y == 0 ? O0 : blockIdx.y == 1 ? O1 : O2;
+    const double* I = blockIdx.y == 0 ? I0 : blockIdx.y == 1 ? I1 : I2;
+    const int64_t n = (int64_t)K * K;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(x / K), j = (int)(x - (int64_t)i * K);
+        O[x] = fs[i] * fs[j] * I[x];
+    }
+}
+
+hipError_t launch_scale_n(double* O0, double* O1, double* O2, const double* I0, const double* I1, const double* I2,
+                          int nmat, const double* fs, int K, hipStream_t st) {
+    const int64_t n = (int64_t)K * K;
+    const int block = 256;
+    const int64_t nb = (n + block - 1) / block;
+    k_scale3<<<dim3((unsigned)(nb < 4096 ? nb : 4096), (unsigned)nmat), block, 0, st>>>(O0, O1, O2, I0, I1, I2, fs, K);
     return hipGetLastError();
 }
 

@@ -29,7 +29,12 @@ bool class_uses_family_a(int qc);
 hipError_t launch_jk_rows(const double* eri, const int2* pairs, int K, int nspin, const double* Dj, const double* Da,
                           const double* Db, double* J, double* Ka, double* Kb, hipStream_t st);
 hipError_t launch_symmetrize(double* M, int K, hipStream_t st);
-hipError_t launch_symmetrize_n(double* M0, double* M1, double* M2, int nmat, int K, hipStream_t st);   // nmat <= 3
+// nmat <= 3; fs != nullptr also scales entry (i, j) by fs[i] fs[j]
+hipError_t launch_symmetrize_n(double* M0, double* M1, double* M2, int nmat, int K, hipStream_t st,
+                               const double* fs = nullptr);
+// O_m = diag(fs) I_m diag(fs) for nmat <= 3 K x K matrices
+hipError_t launch_scale_n(double* O0, double* O1, double* O2, const double* I0, const double* I1, const double* I2,
+                          int nmat, const double* fs, int K, hipStream_t st);
 hipError_t launch_zero(double* p, size_t n, hipStream_t st);
 hipError_t launch_sqrt_clamp(double* p, size_t n, hipStream_t st);   // p = sqrt(max(p, 0))
 hipError_t launch_axpy(double* y, const double* x, size_t n, hipStream_t st);   // y += x

@@ -204,9 +204,21 @@ int emu_fock_direct(EMU_BASIS_ARGS, double threshold, int mixed, double split, c
     QuartetLists ql;
     plan_quartets(c.plan, c.Qsh, threshold, mixed != 0, split, ql, 0, true);   // the device's J/K order
     const size_t K2 = (size_t)K * K;
+    // the device's convention: densities scaled by the function scales
+    // (diag(fs) D diag(fs)), outputs scaled back while symmetrised
+    const double* fs = c.plan.fscale.data();
+    auto scaled = [&](const double* M) {
+        std::vector<double> o(K2);
+        for (int i = 0; i < K; ++i)
+            for (int j = 0; j < K; ++j) o[(size_t)i * K + j] = fs[i] * fs[j] * M[(size_t)i * K + j];
+        return o;
+    };
     std::vector<double> Dj(Pa, Pa + K2);
     if (Pb)
         for (size_t i = 0; i < K2; ++i) Dj[i] += Pb[i];
+    Dj = scaled(Dj.data());
+    const std::vector<double> sPa = scaled(Pa);
+    const std::vector<double> sPb = Pb ? scaled(Pb) : std::vector<double>();
     std::memset(J, 0, K2 * sizeof(double));
     std::memset(Ka, 0, K2 * sizeof(double));
     if (Pb) std::memset(Kb, 0, K2 * sizeof(double));
@@ -215,8 +227,8 @@ int emu_fock_direct(EMU_BASIS_ARGS, double threshold, int mixed, double split, c
     ep.threshold = threshold;
     ep.nspin = Pb ? 2 : 1;
     ep.Dj = Dj.data();
-    ep.Da = Pa;
-    ep.Db = Pb;
+    ep.Da = sPa.data();
+    ep.Db = Pb ? sPb.data() : nullptr;
     ep.J = J;
     ep.Ka = Ka;
     ep.Kb = Kb;
@@ -224,7 +236,7 @@ int emu_fock_direct(EMU_BASIS_ARGS, double threshold, int mixed, double split, c
     auto sym = [&](double* M) {
         for (int i = 0; i < K; ++i)
             for (int j = 0; j <= i; ++j) {
-                const double s = M[i * K + j] + M[j * K + i];
+                const double s = (M[i * K + j] + M[j * K + i]) * fs[i] * fs[j];
                 M[i * K + j] = s;
                 M[j * K + i] = s;
             }
