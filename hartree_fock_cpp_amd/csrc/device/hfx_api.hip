@@ -418,9 +418,10 @@ int upload_lists(const hfx_basis* b, const QuartetLists& ql, SliceTargets target
 }
 
 // Unique-array plans whose total primitive work is at most this many primitive
-// quartets also slice their family-A quartets: such builds (the serving
-// stream's molecules) are set by their longest quartet, while on large ones
-// (benzene / cc-pVDZ: 1.1e8) family-A slices cost more than they balance.
+// quartets also slice their family-A quartets and take the small-plan lane
+// groups (group_bucket_small): such builds (the serving stream's molecules)
+// are set by their longest quartet, while on large ones (benzene / cc-pVDZ:
+// 1.1e8) family-A slices cost more than they balance.
 constexpr long long SMALL_PLAN_WORK = 1LL << 25;
 
 bool small_plan(const BasisPlan& bp, const QuartetLists& ql) {
@@ -445,7 +446,10 @@ int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets ta
     try {
         QuartetLists ql;
         plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql, 0, jk_order);
-        if (target.fam_a == 0 && small_plan(b->plan, ql)) target.fam_a = slice_target();
+        if (target.fam_a == 0 && small_plan(b->plan, ql)) {
+            target.fam_a = slice_target();
+            rebucket(b->plan, ql, group_bucket_small);
+        }
         if (int rc = upload_lists(b, ql, target, dl)) return rc;
     } catch (const std::bad_alloc&) {
         dfree(dl.d);
@@ -497,7 +501,7 @@ int compute_bounds_impl(hfx_basis* b) {
         ql.list[c][0] = diag[c];
         const bool fa = qc_family_a(c);
         for (const uint2& e : diag[c])
-            ++ql.bucket[c][0][fa ? group_bucket(b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim) : NGROUP - 1];
+            ++ql.bucket[c][0][fa ? group_bucket_small(b->plan.hdr[e.x].nprim * b->plan.hdr[e.y].nprim) : NGROUP - 1];
         // diag lists are built in pair order: sort them like plan_quartets does
         std::stable_sort(ql.list[c][0].begin(), ql.list[c][0].end(), [&](const uint2& x, const uint2& y) {
             return b->plan.hdr[x.x].nprim * b->plan.hdr[x.y].nprim > b->plan.hdr[y.x].nprim * b->plan.hdr[y.y].nprim;

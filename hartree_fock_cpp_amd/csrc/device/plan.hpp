@@ -251,12 +251,19 @@ inline void shell_pair_bounds(const BasisPlan& bp, const double* fnb, std::vecto
 }
 
 // Lane-group sizes of family A: a quartet with w = (bra x ket kept primitive
-// pairs) runs on G lanes, G = 64 for w >= 512, 16 for w >= 64, 4 for w >= 8,
-// else 1.  Lists are sorted by w descending, so the buckets are contiguous
-// ranges [G=64 | G=16 | G=4 | G=1].
+// pairs) runs on G lanes, G = 64 for w >= 1024, 16 for w >= 128, 4 for w >= 16,
+// else 1 (measured against 512 / 64 / 8 and 2048 / 256 / 32 on the benzene
+// unique build and the n-octane and (H2O)4 direct builds: 2.09 / 11.2 / 24.3 ms
+// -> 2.00 / 10.6 / 24.2 ms, and 2.07 / 10.4 / 24.3 ms).  Lists are sorted by w
+// descending, so the buckets are contiguous ranges [G=64 | G=16 | G=4 | G=1].
 constexpr int NGROUP = 4;
 constexpr int GROUP_SIZE[NGROUP] = {64, 16, 4, 1};
-inline int group_bucket(int w) { return w >= 512 ? 0 : w >= 64 ? 1 : w >= 8 ? 2 : 3; }
+inline int group_bucket(int w) { return w >= 1024 ? 0 : w >= 128 ? 1 : w >= 16 ? 2 : 3; }
+// Small plans (the Schwarz diagonal, the serving stream's molecules) are set
+// by their longest units rather than by throughput: there wider groups pay
+// (measured: the README molecules' stored SCFs 5-10 % slower with the
+// thresholds above).
+inline int group_bucket_small(int w) { return w >= 512 ? 0 : w >= 64 ? 1 : w >= 8 ? 2 : 3; }
 
 inline bool qc_family_a(int qc) {
     const int pb = qc_bra(qc), pk = qc_ket(qc);
@@ -272,6 +279,19 @@ struct QuartetLists {
         return n;
     }
 };
+
+// Recounts the lane-group buckets of every family-A list with bucket(w) (the
+// lists are sorted by w, so any monotone thresholds give contiguous buckets).
+template <class Bucket>
+inline void rebucket(const BasisPlan& bp, QuartetLists& ql, Bucket bucket) {
+    for (int c = 0; c < NQCLASS; ++c) {
+        if (!qc_family_a(c)) continue;
+        for (int p = 0; p < 2; ++p) {
+            for (int g = 0; g < NGROUP; ++g) ql.bucket[c][p][g] = 0;
+            for (const uint2& e : ql.list[c][p]) ++ql.bucket[c][p][bucket(bp.hdr[e.x].nprim * bp.hdr[e.y].nprim)];
+        }
+    }
+}
 
 // Host threads for planning: one below 2^18 work items (thread start-up
 // would cost more than it saves), else up to 16.
