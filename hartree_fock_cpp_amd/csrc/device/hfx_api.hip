@@ -364,6 +364,9 @@ bool autotune_on() {
     return v == 1;
 }
 SliceTargets jk_slices() { return {slice_target(), slice_target()}; }
+// Direct J/K plans: half the resident-unit target (fewer, longer slices;
+// measured with group_bucket_jk, see there).
+SliceTargets jk_plan_slices() { return {slice_target() / 2, slice_target() / 2}; }
 SliceTargets unique_slices() { return {0, slice_target()}; }
 
 // Uploads the work units of the quartet lists (expand_units with the given
@@ -446,6 +449,7 @@ int make_plan(hfx_basis* b, double thr, int mixed, double split, SliceTargets ta
     try {
         QuartetLists ql;
         plan_quartets(b->plan, b->Qsh, thr, mixed != 0, split, ql, 0, jk_order);
+        if (jk_order) rebucket(b->plan, ql, group_bucket_jk);
         if (target.fam_a == 0 && small_plan(b->plan, ql)) {
             target.fam_a = slice_target();
             rebucket(b->plan, ql, group_bucket_small);
@@ -900,7 +904,8 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
         }
     } else {
         prc = make_plan(b, opts->threshold, mixed, f->opts.mixed_split,
-                        mode == HFX_MODE_STORED ? unique_slices() : jk_slices(), f->plan, mode != HFX_MODE_STORED);
+                        mode == HFX_MODE_STORED ? unique_slices() : jk_plan_slices(), f->plan,
+                        mode != HFX_MODE_STORED);
     }
     if (prc) {
         hfx_fock_destroy(f);
@@ -921,7 +926,7 @@ int hfx_fock_create(hfx_basis* b, const hfx_fock_opts* opts, hfx_fock** out) {
     if (!rc && opts->incremental && mode == HFX_MODE_DIRECT)
         rc = dalloc(&f->d_inc, 8 * K2 + (size_t)b->plan.nshell * b->plan.nshell);
     if (!rc && mode == HFX_MODE_DIRECT) rc = dalloc(&f->d_scl, 3 * K2);
-    f->plan_target = slice_target();
+    f->plan_target = jk_plan_slices().fam_a;   // the direct plan's target (stored plans do not re-plan)
     if (!rc && mode == HFX_MODE_DIRECT && autotune_on() && f->plan_target > 0) {
         f->tune_target[0] = f->plan_target;   // the plan just made
         f->tune_target[1] = std::max<long long>(1, f->plan_target / 4);
