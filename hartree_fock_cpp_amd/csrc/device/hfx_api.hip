@@ -364,9 +364,10 @@ bool autotune_on() {
     return v == 1;
 }
 SliceTargets jk_slices() { return {slice_target(), slice_target()}; }
-// Direct J/K plans: half the resident-unit target (fewer, longer slices;
-// measured with group_bucket_jk, see there).
-SliceTargets jk_plan_slices() { return {slice_target() / 2, slice_target() / 2}; }
+// Direct J/K plans: a quarter of the resident-unit target (fewer, longer
+// slices; with group_bucket_jk measured n-octane / cc-pVDZ 10.1 -> 10.0 ms and
+// (H2O)4 / cc-pVTZ 23.9 -> 23.8 ms against half).
+SliceTargets jk_plan_slices() { return {slice_target() / 4, slice_target() / 4}; }
 SliceTargets unique_slices() { return {0, slice_target()}; }
 
 // Uploads the work units of the quartet lists (expand_units with the given

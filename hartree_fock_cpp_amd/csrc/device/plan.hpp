@@ -266,8 +266,9 @@ inline int group_bucket(int w) { return w >= 1024 ? 0 : w >= 128 ? 1 : w >= 16 ?
 inline int group_bucket_small(int w) { return w >= 512 ? 0 : w >= 64 ? 1 : w >= 8 ? 2 : 3; }
 // Direct J/K plans: narrower groups still (more units per wavefront share the
 // J/K run merging; measured n-octane / cc-pVDZ direct build 10.6 -> 10.2 ms,
-// (H2O)4 / cc-pVTZ 24.2 -> 24.0 ms together with the halved J/K slice target).
-inline int group_bucket_jk(int w) { return w >= 2048 ? 0 : w >= 256 ? 1 : w >= 32 ? 2 : 3; }
+// (H2O)4 / cc-pVTZ 24.2 -> 24.0 ms with 2048 / 256 / 32 and the halved J/K
+// slice target, then 10.2 -> 9.9 ms and 24.0 -> 23.9 ms with 4096 / 512 / 64).
+inline int group_bucket_jk(int w) { return w >= 4096 ? 0 : w >= 512 ? 1 : w >= 64 ? 2 : 3; }
 
 inline bool qc_family_a(int qc) {
     const int pb = qc_bra(qc), pk = qc_ket(qc);
